@@ -1,0 +1,357 @@
+"""ctypes binding of the C ABI in include/guanaco_hip.h (libguanaco_hip.so).
+
+The library must be built in-tree (``python -m guanaco_amd._build``); there
+is no fallback: loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+import numpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libguanaco_hip.so")
+
+GX_OK = 0
+GX_EINVAL = 1
+GX_ENOMEM = 2
+GX_EDEVICE = 3
+GX_EUNSUPPORTED = 4
+
+STORE_F32 = 0
+STORE_F16 = 1
+STORE_BF16 = 2
+STORAGE = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
+
+INPUT_PROJECTIONS = 0
+INPUT_CORRECTED = 1
+INPUT_SINOGRAMS = 2
+
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_int = ctypes.c_int
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class GxError(RuntimeError):
+    """A non-zero status from the C ABI (the reference raises RuntimeError
+    from guanaco::Error)."""
+
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class ReconCfg(ctypes.Structure):
+    _fields_ = [
+        ("num_pixels", _sz), ("num_angles", _sz), ("num_defocus", _sz),
+        ("grid_height", _sz), ("grid_width", _sz),
+        ("pixel_size", ctypes.c_float), ("centre", ctypes.c_float),
+        ("min_defocus", ctypes.c_float), ("max_defocus", ctypes.c_float),
+        ("angles", _f32p), ("gpu_index", _int),
+    ]
+
+
+class VolumeDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_pixels", _sz), ("num_rows", _sz), ("num_angles", _sz),
+        ("num_defocus", _sz), ("depth", _sz), ("width", _sz),
+        ("pixel_size", ctypes.c_float), ("centre", ctypes.c_float),
+        ("min_defocus", ctypes.c_float), ("max_defocus", ctypes.c_float),
+        ("angles", _f32p), ("weights", _f32p),
+        ("storage", _int), ("input_layout", _int),
+        ("defocus", _f64p),
+        ("wavelength", ctypes.c_double), ("Cs", ctypes.c_double),
+        ("Ca", ctypes.c_double), ("Pa", ctypes.c_double),
+        ("phi", ctypes.c_double),
+        ("tilt_chunk", _sz), ("gpu_index", _int),
+    ]
+
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "gx_last_error": [],
+    "gx_version": [],
+    "gx_device_count": [ctypes.POINTER(_int)],
+    "gx_set_device": [_int],
+    "gx_get_device": [ctypes.POINTER(_int)],
+    "gx_device_info": [_int, ctypes.c_char_p, _sz, ctypes.POINTER(_sz),
+                       ctypes.POINTER(_int)],
+    "gx_mem_get_info": [ctypes.POINTER(_sz), ctypes.POINTER(_sz)],
+    "gx_malloc": [ctypes.POINTER(_vp), _sz],
+    "gx_free": [_vp],
+    "gx_memcpy_htod": [_vp, _vp, _sz, _vp],
+    "gx_memcpy_dtoh": [_vp, _vp, _sz, _vp],
+    "gx_memcpy_dtod": [_vp, _vp, _sz, _vp],
+    "gx_fill_u32": [_vp, ctypes.c_uint32, _sz, _vp],
+    "gx_stream_create": [ctypes.POINTER(_vp)],
+    "gx_stream_destroy": [_vp],
+    "gx_stream_synchronize": [_vp],
+    "gx_device_synchronize": [],
+    "gx_event_create": [ctypes.POINTER(_vp)],
+    "gx_event_destroy": [_vp],
+    "gx_event_record": [_vp, _vp],
+    "gx_event_synchronize": [_vp],
+    "gx_event_elapsed_ms": [_vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "gx_graph_capture_begin": [_vp],
+    "gx_graph_capture_end": [_vp, ctypes.POINTER(_vp)],
+    "gx_graph_launch": [_vp, _vp],
+    "gx_graph_destroy": [_vp],
+    "gx_tilt_geometry": [ctypes.POINTER(ReconCfg), _f32p],
+    "gx_recon": [ctypes.POINTER(ReconCfg), _f32p, _f32p],
+    "gx_project": [ctypes.POINTER(ReconCfg), _f32p, _f32p],
+    "gx_filter": [_f32p, _f32p, _sz, _sz, _int],
+    "gx_corr": [_f32p, _f32p, _f32p, _sz, _sz, _sz, _int],
+    "gx_volume_plan_bytes": [ctypes.POINTER(VolumeDesc), ctypes.POINTER(_sz)],
+    "gx_volume_plan_create": [ctypes.POINTER(VolumeDesc), ctypes.POINTER(_vp)],
+    "gx_volume_plan_destroy": [_vp],
+    "gx_volume_run": [_vp, _vp, _vp, _vp],
+    "gx_volume_filter": [_vp, _vp, _vp],
+    "gx_volume_backproject": [_vp, _vp, _vp],
+    "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
+                        ctypes.POINTER(_int)],
+}
+_RESTYPE = {"gx_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libguanaco_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libguanaco_hip.so is not built (run python -m "
+                "guanaco_amd._build); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, _int)
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != GX_OK:
+        msg = lib().gx_last_error().decode(errors="replace")
+        raise GxError(rc, msg or "guanaco_hip error %d" % rc)
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+def device_count():
+    n = _int(0)
+    call("gx_device_count", ctypes.byref(n))
+    return n.value
+
+
+def device_info(dev=0):
+    name = ctypes.create_string_buffer(256)
+    mem = _sz(0)
+    cus = _int(0)
+    call("gx_device_info", dev, name, 256, ctypes.byref(mem), ctypes.byref(cus))
+    return {"name": name.value.decode(), "hbm_bytes": mem.value,
+            "cus": cus.value}
+
+
+def f32p(a):
+    assert a.dtype == numpy.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_f32p)
+
+
+def f64p(a):
+    assert a.dtype == numpy.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_f64p)
+
+
+def recon_cfg(N, T, ndf, H, W, pixel_size, centre, min_defocus, max_defocus,
+              angles, gpu_index=-1):
+    angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
+    cfg = ReconCfg(N, T, ndf, H, W, float(pixel_size), float(centre),
+                   float(min_defocus), float(max_defocus), f32p(angles),
+                   gpu_index)
+    cfg._keep = angles
+    return cfg
+
+
+# ---- device buffers -----------------------------------------------------
+
+
+class DeviceArray:
+    """Owning handle of a device allocation with numpy-side shape/dtype."""
+
+    def __init__(self, shape, dtype=numpy.float32):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = numpy.dtype(dtype)
+        self.nbytes = int(numpy.prod(self.shape, dtype=numpy.int64)) * \
+            self.dtype.itemsize
+        p = _vp()
+        call("gx_malloc", ctypes.byref(p), self.nbytes)
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, stream=None):
+        a = numpy.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        call("gx_memcpy_htod", d.ptr, a.ctypes.data, a.nbytes, stream)
+        if stream is not None:
+            call("gx_stream_synchronize", stream)
+        return d
+
+    def to_host(self, stream=None):
+        out = numpy.empty(self.shape, self.dtype)
+        call("gx_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes, stream)
+        if stream is not None:
+            call("gx_stream_synchronize", stream)
+        return out
+
+    def free(self):
+        if self.ptr:
+            call("gx_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = _vp()
+        call("gx_stream_create", ctypes.byref(p))
+        self.handle = p.value
+
+    def synchronize(self):
+        call("gx_stream_synchronize", self.handle)
+
+    def destroy(self):
+        if self.handle:
+            call("gx_stream_destroy", self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        p = _vp()
+        call("gx_event_create", ctypes.byref(p))
+        self.handle = p.value
+
+    def record(self, stream=None):
+        call("gx_event_record", self.handle,
+             stream.handle if isinstance(stream, Stream) else stream)
+
+    def synchronize(self):
+        call("gx_event_synchronize", self.handle)
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        call("gx_event_elapsed_ms", self.handle, end.handle, ctypes.byref(ms))
+        return ms.value
+
+    def destroy(self):
+        if self.handle:
+            call("gx_event_destroy", self.handle)
+            self.handle = None
+
+
+# ---- whole-volume plans -------------------------------------------------
+
+
+class VolumePlan:
+    """A device-resident reconstruction plan (gx_volume_plan)."""
+
+    def __init__(self, num_pixels, num_rows, angles, depth, width=None,
+                 num_defocus=1, pixel_size=1.0, centre=None, min_defocus=0.0,
+                 max_defocus=0.0, weights=None, storage="f32",
+                 input_layout=INPUT_PROJECTIONS, defocus=None,
+                 wavelength=0.0, Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0,
+                 tilt_chunk=0, gpu_index=-1):
+        self.angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
+        T = self.angles.size
+        self.weights = None if weights is None else numpy.ascontiguousarray(
+            weights, dtype=numpy.float32)
+        self.defocus = None if defocus is None else numpy.ascontiguousarray(
+            defocus, dtype=numpy.float64)
+        width = num_pixels if width is None else width
+        centre = num_pixels / 2.0 if centre is None else centre
+        self.desc = VolumeDesc(
+            num_pixels, num_rows, T, num_defocus, depth, width,
+            float(pixel_size), float(centre), float(min_defocus),
+            float(max_defocus), f32p(self.angles),
+            None if self.weights is None else f32p(self.weights),
+            STORAGE[storage] if isinstance(storage, str) else storage,
+            input_layout,
+            None if self.defocus is None else f64p(self.defocus),
+            float(wavelength), float(Cs), float(Ca), float(Pa), float(phi),
+            tilt_chunk, gpu_index)
+        self.shape = (num_rows, depth, width)
+        self.storage = storage
+        p = _vp()
+        call("gx_volume_plan_create", ctypes.byref(self.desc), ctypes.byref(p))
+        self.handle = p.value
+
+    def footprint(self):
+        n = _sz()
+        call("gx_volume_plan_bytes", ctypes.byref(self.desc), ctypes.byref(n))
+        return n.value
+
+    def run(self, d_in, d_vol, stream=None):
+        call("gx_volume_run", self.handle, d_in.ptr, d_vol.ptr,
+             stream.handle if isinstance(stream, Stream) else stream)
+
+    def filter(self, d_in, stream=None):
+        call("gx_volume_filter", self.handle, d_in.ptr,
+             stream.handle if isinstance(stream, Stream) else stream)
+
+    def backproject(self, d_vol, stream=None):
+        call("gx_volume_backproject", self.handle, d_vol.ptr,
+             stream.handle if isinstance(stream, Stream) else stream)
+
+    def slabs(self):
+        p = _vp()
+        n = _sz()
+        yb = _int()
+        call("gx_volume_slabs", self.handle, ctypes.byref(p), ctypes.byref(n),
+             ctypes.byref(yb))
+        return p.value, n.value, yb.value
+
+    def destroy(self):
+        if getattr(self, "handle", None):
+            call("gx_volume_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def plan_bytes(**kw):
+    """Device footprint of a plan with these arguments (admission)."""
+    angles = numpy.ascontiguousarray(kw.pop("angles"), dtype=numpy.float32)
+    defocus = kw.pop("defocus", None)
+    if defocus is not None:
+        defocus = numpy.ascontiguousarray(defocus, dtype=numpy.float64)
+    storage = kw.pop("storage", "f32")
+    d = VolumeDesc(
+        kw["num_pixels"], kw["num_rows"], angles.size,
+        kw.get("num_defocus", 1), kw["depth"],
+        kw.get("width", kw["num_pixels"]), float(kw.get("pixel_size", 1.0)),
+        float(kw.get("centre", kw["num_pixels"] / 2.0)),
+        float(kw.get("min_defocus", 0.0)), float(kw.get("max_defocus", 0.0)),
+        f32p(angles), None,
+        STORAGE[storage] if isinstance(storage, str) else storage,
+        kw.get("input_layout", INPUT_PROJECTIONS),
+        None if defocus is None else f64p(defocus),
+        0.0, 0.0, 0.0, 0.0, 0.0, kw.get("tilt_chunk", 0), -1)
+    n = _sz()
+    call("gx_volume_plan_bytes", ctypes.byref(d), ctypes.byref(n))
+    return n.value

@@ -1,0 +1,803 @@
+// C ABI of libguanaco_hip.so (declared in include/guanaco_hip.h).
+//
+// Host-side orchestration of the gfx950 kernels: argument checks with the
+// reference's error behaviour (a non-zero status plus a message in place of
+// guanaco::Error), the per-tilt constants computed in the reference's own
+// float/double arithmetic, the ramp-filter table, and device-resident plans
+// for whole volumes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gx_common.hpp"
+#include "gx_fft.hpp"
+#include "gx_kernels.hpp"
+#include "guanaco_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define GX_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(e_ == hipErrorOutOfMemory ? GX_ENOMEM : GX_EDEVICE,        \
+                  "%s:%d %s: %s", __FILE__, __LINE__, #call,                 \
+                  hipGetErrorString(e_));                                    \
+  } while (0)
+
+#define GX_CHECK(cond)                                                         \
+  do {                                                                         \
+    if (!(cond))                                                               \
+      return fail(GX_EINVAL, "%s:%d ASSERT (%s) failed", __FILE__, __LINE__,   \
+                  #cond);                                                      \
+  } while (0)
+
+// Device buffer released on scope exit.
+struct DevBuf {
+  void *p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+bool is_pow2(size_t n) { return n && !(n & (n - 1)); }
+
+size_t next_pow2(size_t n) {
+  size_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+int select_device(int gpu_index) {
+  if (gpu_index >= 0) GX_HIP(hipSetDevice(gpu_index));
+  return GX_OK;
+}
+
+// reconstructor.cpp:29-47 in the reference's float/double arithmetic.
+void tilt_geometry(size_t T, const float *angles, float centre, float *coef) {
+  const double output_scale = M_PI / (double)(2 * T);
+  for (size_t t = 0; t < T; ++t) {
+    const float theta = angles[t];
+    float c = std::cos(theta);
+    float s = std::sin(theta);
+    const float det_x0 = -centre * c;
+    const float det_y0 = -centre * s;
+    const float ray_length = std::sqrt(c * c + s * s);
+    const float d = c * c + s * s;
+    const float offset = (det_y0 * (-s) - det_x0 * c) / d;
+    const float scale =
+        (float)(output_scale * (double)ray_length / (double)std::fabs(d));
+    c = c / d;
+    s = s / d;
+    coef[4 * t + 0] = c;
+    coef[4 * t + 1] = s;
+    coef[4 * t + 2] = offset;
+    coef[4 * t + 3] = scale;
+  }
+}
+
+void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
+                          float *dscale, float *doffset) {
+  const float ds = ndf > 1 ? (float)ndf * ps / (mx - mn) : 0.0f;
+  *dscale = ds;
+  *doffset = -ds * (mn / ps);
+}
+
+// Ramp filter spectrum for rows of N samples zero-padded to L, including the
+// 1/L of the unnormalised inverse: r[j] for j = 0..L/2.
+// L = 2N: the reference's table float(j)/float(N) (filter.cpp:30-45) times
+// 1/(2N). L > 2N (N not a power of two): the reference's 2N-point filter as a
+// space kernel h (|d| <= N-1, all the zero-padded product can reach), then
+// its L-point DFT. Both give the reference's linear convolution.
+std::vector<float> ramp_table(size_t N, size_t L) {
+  std::vector<float> r(L / 2 + 1, 0.0f);
+  if (L == 2 * N) {
+    const float inv_l = 1.0f / (float)L;
+    for (size_t j = 0; j <= N; ++j) r[j] = ((float)j / (float)N) * inv_l;
+    return r;
+  }
+  const size_t M = 2 * N;
+  std::vector<double> cosM(M), cosL(L), h(N);
+  for (size_t i = 0; i < M; ++i) cosM[i] = std::cos(2.0 * M_PI * i / M);
+  for (size_t i = 0; i < L; ++i) cosL[i] = std::cos(2.0 * M_PI * i / L);
+  for (size_t d = 0; d < N; ++d) {
+    double acc = 0.0;
+    for (size_t k = 0; k < M; ++k) {
+      const size_t kk = k <= N ? k : M - k;
+      const double R = (double)((float)kk / (float)N);
+      acc += R * cosM[(k * d) % M];
+    }
+    h[d] = acc / (double)M;
+  }
+  for (size_t j = 0; j <= L / 2; ++j) {
+    double acc = h[0];
+    for (size_t d = 1; d < N; ++d) acc += 2.0 * h[d] * cosL[(j * d) % L];
+    r[j] = (float)(acc / (double)L);
+  }
+  return r;
+}
+
+size_t filter_length(size_t N) {
+  size_t L = next_pow2(2 * N - 1);
+  return L < 8 ? 8 : L;
+}
+
+}  // namespace
+
+namespace gx {
+
+const float2 *twiddles(hipError_t *err) {
+  static std::mutex mu;
+  static float2 *tables[64] = {};
+  int dev = 0;
+  *err = hipGetDevice(&dev);
+  if (*err != hipSuccess) return nullptr;
+  if (dev < 0 || dev >= 64) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (tables[dev] == nullptr) {
+    std::vector<float2> h(kTwN);
+    for (int j = 0; j < kTwN; ++j) {
+      const double a = 2.0 * M_PI * (double)j / (double)kTwN;
+      h[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    float2 *d = nullptr;
+    *err = hipMalloc(&d, sizeof(float2) * kTwN);
+    if (*err != hipSuccess) return nullptr;
+    *err = hipMemcpy(d, h.data(), sizeof(float2) * kTwN, hipMemcpyHostToDevice);
+    if (*err != hipSuccess) {
+      (void)hipFree(d);
+      return nullptr;
+    }
+    tables[dev] = d;  // held for the life of the process
+  }
+  *err = hipSuccess;
+  return tables[dev];
+}
+
+}  // namespace gx
+
+using namespace gx;
+
+// ------------------------------------------------------------ status ------
+
+extern "C" const char *gx_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gx_version(void) { return 1; }
+
+extern "C" int gx_device_count(int *count) {
+  GX_CHECK(count != nullptr);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return GX_OK;
+}
+
+extern "C" int gx_set_device(int device) {
+  GX_HIP(hipSetDevice(device));
+  return GX_OK;
+}
+
+extern "C" int gx_get_device(int *device) {
+  GX_CHECK(device != nullptr);
+  GX_HIP(hipGetDevice(device));
+  return GX_OK;
+}
+
+extern "C" int gx_device_info(int device, char *name, size_t name_len,
+                              size_t *total_mem, int *num_cus) {
+  hipDeviceProp_t p;
+  GX_HIP(hipGetDeviceProperties(&p, device));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (total_mem) *total_mem = p.totalGlobalMem;
+  if (num_cus) *num_cus = p.multiProcessorCount;
+  return GX_OK;
+}
+
+extern "C" int gx_mem_get_info(size_t *free_bytes, size_t *total_bytes) {
+  GX_CHECK(free_bytes && total_bytes);
+  GX_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return GX_OK;
+}
+
+// ------------------------------------------------- memory and streams -----
+
+extern "C" int gx_malloc(void **ptr, size_t bytes) {
+  GX_CHECK(ptr != nullptr);
+  GX_HIP(hipMalloc(ptr, bytes ? bytes : 16));
+  return GX_OK;
+}
+
+extern "C" int gx_free(void *ptr) {
+  if (ptr) GX_HIP(hipFree(ptr));
+  return GX_OK;
+}
+
+extern "C" int gx_memcpy_htod(void *dst, const void *src, size_t bytes,
+                              void *stream) {
+  GX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice,
+                        (hipStream_t)stream));
+  if (stream == nullptr) GX_HIP(hipStreamSynchronize(nullptr));
+  return GX_OK;
+}
+
+extern "C" int gx_memcpy_dtoh(void *dst, const void *src, size_t bytes,
+                              void *stream) {
+  GX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost,
+                        (hipStream_t)stream));
+  if (stream == nullptr) GX_HIP(hipStreamSynchronize(nullptr));
+  return GX_OK;
+}
+
+extern "C" int gx_memcpy_dtod(void *dst, const void *src, size_t bytes,
+                              void *stream) {
+  GX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice,
+                        (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_fill_u32(void *dst, uint32_t value, size_t count,
+                           void *stream) {
+  GX_HIP(launch_fill_u32((uint32_t *)dst, value, count, (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_stream_create(void **stream) {
+  GX_CHECK(stream != nullptr);
+  hipStream_t s;
+  GX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return GX_OK;
+}
+
+extern "C" int gx_stream_destroy(void *stream) {
+  if (stream) GX_HIP(hipStreamDestroy((hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_stream_synchronize(void *stream) {
+  GX_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_device_synchronize(void) {
+  GX_HIP(hipDeviceSynchronize());
+  return GX_OK;
+}
+
+extern "C" int gx_event_create(void **event) {
+  GX_CHECK(event != nullptr);
+  hipEvent_t e;
+  GX_HIP(hipEventCreate(&e));
+  *event = e;
+  return GX_OK;
+}
+
+extern "C" int gx_event_destroy(void *event) {
+  if (event) GX_HIP(hipEventDestroy((hipEvent_t)event));
+  return GX_OK;
+}
+
+extern "C" int gx_event_record(void *event, void *stream) {
+  GX_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_event_synchronize(void *event) {
+  GX_HIP(hipEventSynchronize((hipEvent_t)event));
+  return GX_OK;
+}
+
+extern "C" int gx_event_elapsed_ms(void *start, void *stop, float *ms) {
+  GX_CHECK(ms != nullptr);
+  GX_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return GX_OK;
+}
+
+extern "C" int gx_graph_capture_begin(void *stream) {
+  GX_CHECK(stream != nullptr);
+  GX_HIP(hipStreamBeginCapture((hipStream_t)stream,
+                               hipStreamCaptureModeThreadLocal));
+  return GX_OK;
+}
+
+extern "C" int gx_graph_capture_end(void *stream, void **graph_exec) {
+  GX_CHECK(stream != nullptr && graph_exec != nullptr);
+  hipGraph_t g = nullptr;
+  GX_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  GX_HIP(e);
+  *graph_exec = ge;
+  return GX_OK;
+}
+
+extern "C" int gx_graph_launch(void *graph_exec, void *stream) {
+  GX_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_graph_destroy(void *graph_exec) {
+  if (graph_exec) GX_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return GX_OK;
+}
+
+// ------------------------------------ reference-equivalent entry points ---
+
+static int check_recon_cfg(const gx_recon_cfg *c) {
+  GX_CHECK(c != nullptr);
+  // config.h:66-75
+  GX_CHECK(c->num_pixels > 0);
+  GX_CHECK(c->num_angles > 0);
+  GX_CHECK(c->num_defocus > 0);
+  GX_CHECK(c->grid_height > 0);
+  GX_CHECK(c->grid_width > 0);
+  GX_CHECK(c->pixel_size > 0);
+  GX_CHECK(c->min_defocus <= c->max_defocus);
+  GX_CHECK(c->angles != nullptr);
+  if (filter_length(c->num_pixels) > 8192)
+    return fail(GX_EUNSUPPORTED,
+                "num_pixels %zu exceeds the gfx950 filter limit (4096)",
+                c->num_pixels);
+  return GX_OK;
+}
+
+extern "C" int gx_tilt_geometry(const gx_recon_cfg *cfg, float *coef) {
+  GX_CHECK(cfg != nullptr && cfg->angles != nullptr && coef != nullptr);
+  GX_CHECK(cfg->num_angles > 0);
+  tilt_geometry(cfg->num_angles, cfg->angles, cfg->centre, coef);
+  return GX_OK;
+}
+
+// Shared by gx_recon and gx_project: slab (ndf x T x N, filtered, device)
+// -> rec (H x W, device).
+static int bp_slice(const gx_recon_cfg *c, const float *d_slab, float *d_rec,
+                    hipStream_t s) {
+  const size_t T = c->num_angles;
+  std::vector<float> coef(4 * T);
+  tilt_geometry(T, c->angles, c->centre, coef.data());
+  DevBuf d_coef;
+  GX_HIP(d_coef.alloc(sizeof(float) * 4 * T));
+  GX_HIP(hipMemcpyAsync(d_coef.p, coef.data(), sizeof(float) * 4 * T,
+                        hipMemcpyHostToDevice, s));
+  BpArgs a{};
+  a.slab = d_slab;
+  a.coef = d_coef.as<float4>();
+  a.tmax = nullptr;
+  a.nrm_mul = 1.0f;
+  defocus_coefficients(c->num_defocus, c->pixel_size, c->min_defocus,
+                       c->max_defocus, &a.dscale, &a.doffset);
+  a.ndf = (int)c->num_defocus;
+  a.N = (int)c->num_pixels;
+  a.T = (int)T;
+  a.NYB = 1;
+  a.W = (int)c->grid_width;
+  a.Z = (int)c->grid_height;
+  a.Y = 1;
+  a.vol = d_rec;
+  GX_HIP(launch_bp(a, 1, kF32, s));
+  GX_HIP(hipStreamSynchronize(s));
+  return GX_OK;
+}
+
+extern "C" int gx_project(const gx_recon_cfg *cfg, const float *filtered,
+                          float *rec) {
+  int rc = check_recon_cfg(cfg);
+  if (rc) return rc;
+  GX_CHECK(filtered != nullptr && rec != nullptr);
+  rc = select_device(cfg->gpu_index);
+  if (rc) return rc;
+  const size_t n_sino = cfg->num_defocus * cfg->num_angles * cfg->num_pixels;
+  const size_t n_rec = cfg->grid_height * cfg->grid_width;
+  DevBuf d_sino, d_rec;
+  GX_HIP(d_sino.alloc(sizeof(float) * n_sino));
+  GX_HIP(d_rec.alloc(sizeof(float) * n_rec));
+  GX_HIP(hipMemcpy(d_sino.p, filtered, sizeof(float) * n_sino,
+                   hipMemcpyHostToDevice));
+  rc = bp_slice(cfg, d_sino.as<float>(), d_rec.as<float>(), nullptr);
+  if (rc) return rc;
+  GX_HIP(hipMemcpy(rec, d_rec.p, sizeof(float) * n_rec, hipMemcpyDeviceToHost));
+  return GX_OK;
+}
+
+// Rows (nrows x N, device) -> filtered rows (nrows x N, device), F32.
+static int filter_rows_dev(const float *d_in, float *d_out, size_t nrows,
+                           size_t N, const float *d_wscale_ones,
+                           hipStream_t s) {
+  const size_t L = filter_length(N);
+  std::vector<float> r = ramp_table(N, L);
+  DevBuf d_ramp;
+  GX_HIP(d_ramp.alloc(sizeof(float) * r.size()));
+  GX_HIP(hipMemcpyAsync(d_ramp.p, r.data(), sizeof(float) * r.size(),
+                        hipMemcpyHostToDevice, s));
+  // rows as (ndf = 1, T = nrows, Y = 1): slab layout [t][x] = input layout
+  GX_HIP(launch_filter_rows(d_in, N, 0, 0, (int)N, (int)L, 1, (int)nrows, 1,
+                            d_ramp.as<float>(), d_wscale_ones, nullptr, 1.0f,
+                            kF32, d_out, 1, 1, s));
+  GX_HIP(hipStreamSynchronize(s));
+  return GX_OK;
+}
+
+extern "C" int gx_filter(const float *rows, float *out, size_t nrows,
+                         size_t num_pixels, int gpu_index) {
+  GX_CHECK(rows != nullptr && out != nullptr);
+  GX_CHECK(num_pixels > 0 && nrows > 0);
+  if (filter_length(num_pixels) > 8192)
+    return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", num_pixels);
+  int rc = select_device(gpu_index);
+  if (rc) return rc;
+  const size_t n = nrows * num_pixels;
+  DevBuf d_in, d_out, d_w;
+  GX_HIP(d_in.alloc(sizeof(float) * n));
+  GX_HIP(d_out.alloc(sizeof(float) * n));
+  GX_HIP(d_w.alloc(sizeof(float) * nrows));
+  std::vector<float> ones(nrows, 1.0f);
+  GX_HIP(hipMemcpy(d_in.p, rows, sizeof(float) * n, hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(d_w.p, ones.data(), sizeof(float) * nrows,
+                   hipMemcpyHostToDevice));
+  rc = filter_rows_dev(d_in.as<float>(), d_out.as<float>(), nrows, num_pixels,
+                       d_w.as<float>(), nullptr);
+  if (rc) return rc;
+  GX_HIP(hipMemcpy(out, d_out.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return GX_OK;
+}
+
+extern "C" int gx_recon(const gx_recon_cfg *cfg, const float *sino,
+                        float *rec) {
+  int rc = check_recon_cfg(cfg);
+  if (rc) return rc;
+  GX_CHECK(sino != nullptr && rec != nullptr);
+  rc = select_device(cfg->gpu_index);
+  if (rc) return rc;
+  const size_t T = cfg->num_angles, N = cfg->num_pixels;
+  const size_t nrows = cfg->num_defocus * T;
+  const size_t n_sino = nrows * N;
+  const size_t n_rec = cfg->grid_height * cfg->grid_width;
+  DevBuf d_sino, d_filt, d_rec, d_w;
+  GX_HIP(d_sino.alloc(sizeof(float) * n_sino));
+  GX_HIP(d_filt.alloc(sizeof(float) * n_sino));
+  GX_HIP(d_rec.alloc(sizeof(float) * n_rec));
+  GX_HIP(d_w.alloc(sizeof(float) * nrows));
+  std::vector<float> ones(nrows, 1.0f);
+  // reconstructor.cpp:105: the input is copied, never modified
+  GX_HIP(hipMemcpy(d_sino.p, sino, sizeof(float) * n_sino,
+                   hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(d_w.p, ones.data(), sizeof(float) * nrows,
+                   hipMemcpyHostToDevice));
+  rc = filter_rows_dev(d_sino.as<float>(), d_filt.as<float>(), nrows, N,
+                       d_w.as<float>(), nullptr);
+  if (rc) return rc;
+  rc = bp_slice(cfg, d_filt.as<float>(), d_rec.as<float>(), nullptr);
+  if (rc) return rc;
+  GX_HIP(hipMemcpy(rec, d_rec.p, sizeof(float) * n_rec, hipMemcpyDeviceToHost));
+  return GX_OK;
+}
+
+static int check_corr_sizes(size_t X, size_t Y) {
+  if (!is_pow2(X) || X < 8 || X > 4096 || !is_pow2(Y) || Y < 4 || Y > 4096)
+    return fail(GX_EUNSUPPORTED,
+                "CTF correction on gfx950 needs power-of-two sizes, "
+                "8 <= xsize <= 4096 and 4 <= ysize <= 4096 (got %zu x %zu)",
+                X, Y);
+  return GX_OK;
+}
+
+extern "C" int gx_corr(const float *image, const float *ctf, float *rec,
+                       size_t xsize, size_t ysize, size_t num_ctf,
+                       int gpu_index) {
+  // ext/correct.h:40-57 and correct.cpp:37-39
+  GX_CHECK(image != nullptr && ctf != nullptr && rec != nullptr);
+  GX_CHECK(xsize > 0);
+  GX_CHECK(ysize > 0);
+  GX_CHECK(num_ctf > 0);
+  int rc = check_corr_sizes(xsize, ysize);
+  if (rc) return rc;
+  rc = select_device(gpu_index);
+  if (rc) return rc;
+  const int X = (int)xsize, Y = (int)ysize, ndf = (int)num_ctf;
+  const int XB = X / 2 / CB + 1;
+  const size_t npix = xsize * ysize;
+  const size_t nspec = (size_t)XB * CB * Y;
+  DevBuf d_img, d_ctf, d_mask, d_F, d_G, d_out;
+  GX_HIP(d_img.alloc(sizeof(float) * npix));
+  GX_HIP(d_ctf.alloc(sizeof(float2) * npix * num_ctf));
+  GX_HIP(d_mask.alloc(nspec * num_ctf));
+  GX_HIP(d_F.alloc(sizeof(float2) * nspec));
+  GX_HIP(d_G.alloc(sizeof(float2) * nspec * num_ctf));
+  GX_HIP(d_out.alloc(sizeof(float) * npix * num_ctf));
+  GX_HIP(hipMemcpy(d_img.p, image, sizeof(float) * npix, hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(d_ctf.p, ctf, sizeof(float2) * npix * num_ctf,
+                   hipMemcpyHostToDevice));
+  hipStream_t s = nullptr;
+  GX_HIP(launch_mask_from_ctf(d_mask.as<int8_t>(), d_ctf.as<float2>(), X, Y, XB,
+                              ndf, s));
+  GX_HIP(launch_rows_r2c(d_img.as<float>(), npix, X, Y, XB, 1, d_F.as<float2>(),
+                         nullptr, 0, s));
+  GX_HIP(launch_cols(d_F.as<float2>(), d_mask.as<int8_t>(), d_G.as<float2>(), Y,
+                     XB, ndf, 1, s));
+  GX_HIP(launch_rows_c2r(d_G.as<float2>(), X, Y, XB, ndf, 1, d_out.as<float>(),
+                         s));
+  GX_HIP(hipStreamSynchronize(s));
+  GX_HIP(hipMemcpy(rec, d_out.p, sizeof(float) * npix * num_ctf,
+                   hipMemcpyDeviceToHost));
+  return GX_OK;
+}
+
+// ------------------------------------------------- whole-volume plans ----
+
+struct gx_volume_plan {
+  int device = 0;
+  int X = 0, Y = 0, T = 0, ndf = 0, Z = 0, W = 0;
+  int storage = 0, layout = 0;
+  int YB = 0, NYB = 0, XB = 0, L = 0, chunk = 0;
+  bool ctf = false;
+  float dscale = 0, doffset = 0, nrm_mul = 1;
+  size_t slab_bytes = 0, total_bytes = 0;
+  DevBuf coef, wscale, tmax, ramp, mask, defocus, F, G, slab;
+};
+
+namespace {
+
+struct PlanSizes {
+  int YB, NYB, XB, L, chunk;
+  bool ctf;
+  size_t slab_bytes, mask_bytes, f_bytes, g_bytes, small_bytes, total;
+};
+
+int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
+  GX_CHECK(d != nullptr);
+  GX_CHECK(d->num_pixels > 0 && d->num_rows > 0 && d->num_angles > 0);
+  GX_CHECK(d->num_defocus > 0 && d->depth > 0 && d->width > 0);
+  GX_CHECK(d->pixel_size > 0);
+  GX_CHECK(d->min_defocus <= d->max_defocus);
+  GX_CHECK(d->angles != nullptr);
+  GX_CHECK(d->storage >= GX_STORE_F32 && d->storage <= GX_STORE_BF16);
+  GX_CHECK(d->input_layout >= GX_INPUT_PROJECTIONS &&
+           d->input_layout <= GX_INPUT_SINOGRAMS);
+  const size_t X = d->num_pixels, Y = d->num_rows, T = d->num_angles;
+  const size_t ndf = d->num_defocus;
+  ps->ctf = d->input_layout == GX_INPUT_PROJECTIONS && d->defocus != nullptr;
+  if (d->input_layout == GX_INPUT_PROJECTIONS && !ps->ctf && ndf != 1)
+    return fail(GX_EINVAL,
+                "num_defocus must be 1 for uncorrected projections without "
+                "CTF parameters");
+  if (ps->ctf) {
+    int rc = check_corr_sizes(X, Y);
+    if (rc) return rc;
+  } else if (filter_length(X) > 8192) {
+    return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", X);
+  }
+  ps->YB = d->storage == GX_STORE_F32 ? 8 : 16;
+  ps->NYB = (int)((Y + ps->YB - 1) / ps->YB);
+  ps->XB = (int)(X / 2 / CB + 1);
+  ps->L = (int)(ps->ctf ? 2 * X : filter_length(X));
+  const size_t es = d->storage == GX_STORE_F32 ? 4 : 2;
+  ps->slab_bytes = ndf * T * (size_t)ps->NYB * ps->YB * X * es;
+  const size_t nspec = (size_t)ps->XB * CB * Y;
+  ps->mask_bytes = ps->ctf ? nspec * ndf : 0;
+  size_t chunk = 0;
+  if (ps->ctf) {
+    const size_t per_tilt = nspec * sizeof(float2) * (1 + ndf);
+    chunk = d->tilt_chunk;
+    if (chunk == 0) {
+      const size_t budget = (size_t)4 << 30;
+      chunk = budget / per_tilt;
+      if (chunk < 1) chunk = 1;
+    }
+    if (chunk > T) chunk = T;
+    ps->f_bytes = nspec * sizeof(float2) * chunk;
+    ps->g_bytes = nspec * sizeof(float2) * ndf * chunk;
+  } else {
+    ps->f_bytes = ps->g_bytes = 0;
+  }
+  ps->chunk = (int)chunk;
+  ps->small_bytes = T * (sizeof(float4) + 2 * sizeof(float)) +
+                    sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf;
+  ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
+              ps->small_bytes;
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" int gx_volume_plan_bytes(const gx_volume_desc *desc,
+                                    size_t *bytes) {
+  GX_CHECK(bytes != nullptr);
+  PlanSizes ps;
+  int rc = plan_sizes(desc, &ps);
+  if (rc) return rc;
+  *bytes = ps.total;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
+                                     gx_volume_plan **out) {
+  GX_CHECK(out != nullptr);
+  *out = nullptr;
+  PlanSizes ps;
+  int rc = plan_sizes(d, &ps);
+  if (rc) return rc;
+  rc = select_device(d->gpu_index);
+  if (rc) return rc;
+  hipError_t terr;
+  twiddles(&terr);  // build the table outside any capture
+  GX_HIP(terr);
+
+  auto *p = new gx_volume_plan();
+  std::unique_ptr<gx_volume_plan> guard(p);
+  GX_HIP(hipGetDevice(&p->device));
+  p->X = (int)d->num_pixels;
+  p->Y = (int)d->num_rows;
+  p->T = (int)d->num_angles;
+  p->ndf = (int)d->num_defocus;
+  p->Z = (int)d->depth;
+  p->W = (int)d->width;
+  p->storage = d->storage;
+  p->layout = d->input_layout;
+  p->YB = ps.YB;
+  p->NYB = ps.NYB;
+  p->XB = ps.XB;
+  p->L = ps.L;
+  p->chunk = ps.chunk;
+  p->ctf = ps.ctf;
+  p->slab_bytes = ps.slab_bytes;
+  p->total_bytes = ps.total;
+  p->nrm_mul = ps.ctf ? (float)d->num_pixels * (float)d->num_rows : 1.0f;
+  defocus_coefficients(d->num_defocus, d->pixel_size, d->min_defocus,
+                       d->max_defocus, &p->dscale, &p->doffset);
+
+  const size_t T = d->num_angles;
+  std::vector<float> coef(4 * T), w(T, 1.0f);
+  tilt_geometry(T, d->angles, d->centre, coef.data());
+  if (d->weights)
+    for (size_t t = 0; t < T; ++t) w[t] = d->weights[t];
+  std::vector<float> ramp = ramp_table(d->num_pixels, (size_t)ps.L);
+
+  GX_HIP(p->coef.alloc(sizeof(float4) * T));
+  GX_HIP(p->wscale.alloc(sizeof(float) * T));
+  GX_HIP(p->tmax.alloc(sizeof(uint32_t) * T));
+  GX_HIP(p->ramp.alloc(sizeof(float) * ramp.size()));
+  GX_HIP(p->slab.alloc(ps.slab_bytes));
+  GX_HIP(hipMemcpy(p->coef.p, coef.data(), sizeof(float4) * T,
+                   hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(p->wscale.p, w.data(), sizeof(float) * T,
+                   hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpy(p->ramp.p, ramp.data(), sizeof(float) * ramp.size(),
+                   hipMemcpyHostToDevice));
+  // rows past Y in the last row block are read by the backprojector and
+  // discarded; keep them finite
+  GX_HIP(launch_fill_u32(p->slab.as<uint32_t>(), 0u, ps.slab_bytes / 4,
+                         nullptr));
+  if (ps.ctf) {
+    GX_HIP(p->defocus.alloc(sizeof(double) * d->num_defocus));
+    GX_HIP(hipMemcpy(p->defocus.p, d->defocus,
+                     sizeof(double) * d->num_defocus, hipMemcpyHostToDevice));
+    GX_HIP(p->mask.alloc(ps.mask_bytes));
+    GX_HIP(p->F.alloc(ps.f_bytes));
+    GX_HIP(p->G.alloc(ps.g_bytes));
+    GX_HIP(launch_ctf_mask(p->mask.as<int8_t>(), p->X, p->Y, p->XB, p->ndf,
+                           p->defocus.as<double>(), d->wavelength, d->Cs,
+                           d->Ca, d->Pa, d->phi, d->pixel_size, nullptr));
+  }
+  GX_HIP(hipDeviceSynchronize());
+  *out = guard.release();
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_destroy(gx_volume_plan *plan) {
+  delete plan;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
+                                void *stream) {
+  GX_CHECK(p != nullptr && d_in != nullptr);
+  hipStream_t s = (hipStream_t)stream;
+  const bool reduced = p->storage != GX_STORE_F32;
+  uint32_t *tmax = reduced ? p->tmax.as<uint32_t>() : nullptr;
+  if (reduced) GX_HIP(launch_fill_u32(tmax, 0u, (size_t)p->T, s));
+  const size_t X = p->X, Y = p->Y, T = p->T, ndf = p->ndf;
+  if (p->ctf) {
+    for (int t0 = 0; t0 < p->T; t0 += p->chunk) {
+      const int tc = p->T - t0 < p->chunk ? p->T - t0 : p->chunk;
+      GX_HIP(launch_rows_r2c(d_in + (size_t)t0 * Y * X, Y * X, p->X, p->Y,
+                             p->XB, tc, p->F.as<float2>(), tmax, t0, s));
+      GX_HIP(launch_cols(p->F.as<float2>(), p->mask.as<int8_t>(),
+                         p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s));
+      GX_HIP(launch_rows_c2r_filter(
+          p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
+          p->ramp.as<float>(), p->wscale.as<float>(), tmax, p->nrm_mul,
+          p->storage, p->slab.p, p->NYB, p->YB, s));
+    }
+  } else {
+    size_t st, sk, sy;
+    if (p->layout == GX_INPUT_SINOGRAMS) {
+      st = X;
+      sk = T * X;
+      sy = ndf * T * X;
+    } else {  // (T, [ndf,] Y, X)
+      st = ndf * Y * X;
+      sk = Y * X;
+      sy = X;
+    }
+    if (reduced)
+      GX_HIP(launch_absmax_rows(d_in, st, sk, sy, p->X, p->Y, p->T, p->ndf,
+                                tmax, s));
+    GX_HIP(launch_filter_rows(d_in, st, sk, sy, p->X, p->L, p->Y, p->T, p->ndf,
+                              p->ramp.as<float>(), p->wscale.as<float>(), tmax,
+                              p->nrm_mul, p->storage, p->slab.p, p->NYB, p->YB,
+                              s));
+  }
+  return GX_OK;
+}
+
+extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
+                                     void *stream) {
+  GX_CHECK(p != nullptr && d_vol != nullptr);
+  BpArgs a{};
+  a.slab = p->slab.p;
+  a.coef = p->coef.as<float4>();
+  a.tmax = p->storage != GX_STORE_F32 ? p->tmax.as<uint32_t>() : nullptr;
+  a.nrm_mul = p->nrm_mul;
+  a.dscale = p->dscale;
+  a.doffset = p->doffset;
+  a.ndf = p->ndf;
+  a.N = p->X;
+  a.T = p->T;
+  a.NYB = p->NYB;
+  a.W = p->W;
+  a.Z = p->Z;
+  a.Y = p->Y;
+  a.vol = d_vol;
+  GX_HIP(launch_bp(a, p->YB, p->storage, (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_volume_run(gx_volume_plan *p, const float *d_in,
+                             float *d_vol, void *stream) {
+  int rc = gx_volume_filter(p, d_in, stream);
+  if (rc) return rc;
+  return gx_volume_backproject(p, d_vol, stream);
+}
+
+extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
+                               int *rows_per_block) {
+  GX_CHECK(p != nullptr);
+  if (d_slab) *d_slab = p->slab.p;
+  if (bytes) *bytes = p->slab_bytes;
+  if (rows_per_block) *rows_per_block = p->YB;
+  return GX_OK;
+}

@@ -1,0 +1,576 @@
+// CTF phase-flip correction and ramp filtering on gfx950.
+//
+// Reference semantics (src/libguanaco/correct.cpp:30-90, filter.cpp:30-97):
+//   corrected_k = Re(IFFT2_unnorm(FFT2(image) * -sign(Im ctf_k)))
+//   filtered    = first N of Re(IFFT_2N(ramp * FFT_2N(pad(row)))) / 2N
+// Here the 2D transforms are real-to-complex: a Hermitian half spectrum
+// (X/2 + 1 columns, padded to a multiple of CB) is computed once per tilt,
+// flipped with a per-defocus mask, inverse transformed along columns, and
+// the final row pass fuses the complex-to-real row transform with the ramp
+// filter, the per-tilt output scale and the store into the backprojection
+// layout. Two real rows travel as one complex row (a + i b) through every
+// row transform, and the ramp (real and even) keeps them separable.
+#include "gx_common.hpp"
+#include "gx_fft.hpp"
+#include "gx_kernels.hpp"
+
+namespace gx {
+
+// ---------------------------------------------------------------- helpers --
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t v, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p,
+                     v, n);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------- CTF masks --
+
+// The reference stores H = exp(-i(chi - phi)) as complex64 (correct.py:192)
+// and flips by -sign(Im H) (correct.h:30-39). Im H = sin(-(chi - phi)) is
+// evaluated in FP64; its float cast is +-0 exactly when |s| <= 2^-150.
+__device__ __forceinline__ int flip_sign(double s) {
+  const double tiny = 0x1p-150;
+  return -((s > tiny) - (s < -tiny));
+}
+
+// ctf.h:88-101 (get_r_and_theta) and ctf.h:233-238 (get_chi), FP64.
+__device__ double chi_at(int x, int y, int w, int h, double inv_ps, double l,
+                         double df, double Cs, double Ca, double Pa) {
+  const size_t w2 = (size_t)w / 2, h2 = (size_t)h / 2;
+  const double dx =
+      ((double)(((size_t)x + w2) % (size_t)w) - (double)w2) / (double)w;
+  const double dy =
+      ((double)(((size_t)y + h2) % (size_t)h) - (double)h2) / (double)h;
+  const double q = sqrt((dx * dx + dy * dy) * inv_ps * inv_ps);
+  double d = df;
+  if (Ca != 0.0) d = df + Ca * cos(2 * (atan2(dy, dx) - Pa));
+  return M_PI * (Cs * l * l * l * q * q * q * q / 2 - l * d * q * q);
+}
+
+// mask[k][cb][ky][ci] = M(k) + M(-k) in {-2..2}: twice the Hermitian part of
+// the flip, which is what Re(IFFT2(...)) keeps. Columns past X/2 are 0.
+__global__ void k_ctf_mask(int8_t *__restrict__ mask, int X, int Y, int XB,
+                           int ndf, const double *__restrict__ defocus,
+                           double l, double Cs, double Ca, double Pa,
+                           double phi, double inv_ps) {
+  const size_t total = (size_t)ndf * XB * Y * CB;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int ci = (int)(e % CB);
+  const int ky = (int)((e / CB) % Y);
+  const int cb = (int)((e / ((size_t)CB * Y)) % XB);
+  const int k = (int)(e / ((size_t)CB * Y * XB));
+  const int kx = cb * CB + ci;
+  int v = 0;
+  if (kx <= X / 2) {
+    const double df = defocus[k];
+    const double c1 = chi_at(kx, ky, X, Y, inv_ps, l, df, Cs, Ca, Pa);
+    const int m1 = flip_sign(sin(-(c1 - phi)));
+    int m2 = m1;
+    if (Ca != 0.0) {
+      const int mx = (X - kx) & (X - 1), my = (Y - ky) & (Y - 1);
+      const double c2 = chi_at(mx, my, X, Y, inv_ps, l, df, Cs, Ca, Pa);
+      m2 = flip_sign(sin(-(c2 - phi)));
+    }
+    v = m1 + m2;
+  }
+  mask[e] = (int8_t)v;
+}
+
+hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
+                           const double *d_defocus, double l, double Cs,
+                           double Ca, double Pa, double phi, double ps,
+                           hipStream_t s) {
+  const size_t total = (size_t)ndf * XB * Y * CB;
+  hipLaunchKernelGGL(k_ctf_mask, dim3((unsigned)((total + 255) / 256)),
+                     dim3(256), 0, s, mask, X, Y, XB, ndf, d_defocus, l, Cs,
+                     Ca, Pa, phi, 1.0 / ps);
+  return hipGetLastError();
+}
+
+// Same mask from a caller-supplied complex64 CTF stack (ndf, Y, X), as the
+// reference's corr() receives it (ext/correct.h:32-76).
+__global__ void k_mask_from_ctf(int8_t *__restrict__ mask,
+                                const float2 *__restrict__ ctf, int X, int Y,
+                                int XB, int ndf) {
+  const size_t total = (size_t)ndf * XB * Y * CB;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int ci = (int)(e % CB);
+  const int ky = (int)((e / CB) % Y);
+  const int cb = (int)((e / ((size_t)CB * Y)) % XB);
+  const int k = (int)(e / ((size_t)CB * Y * XB));
+  const int kx = cb * CB + ci;
+  int v = 0;
+  if (kx <= X / 2) {
+    const float2 *c = ctf + (size_t)k * X * Y;
+    const float a = c[(size_t)ky * X + kx].y;
+    const int mx = (X - kx) & (X - 1), my = (Y - ky) & (Y - 1);
+    const float b = c[(size_t)my * X + mx].y;
+    v = -((0.0f < a) - (a < 0.0f)) - ((0.0f < b) - (b < 0.0f));
+  }
+  mask[e] = (int8_t)v;
+}
+
+hipError_t launch_mask_from_ctf(int8_t *mask, const float2 *ctf, int X, int Y,
+                                int XB, int ndf, hipStream_t s) {
+  const size_t total = (size_t)ndf * XB * Y * CB;
+  hipLaunchKernelGGL(k_mask_from_ctf, dim3((unsigned)((total + 255) / 256)),
+                     dim3(256), 0, s, mask, ctf, X, Y, XB, ndf);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------- pass 1: rows, R2C -------
+
+// Four image rows per workgroup (two complex pairs). Writes the half
+// spectrum F[t][cb][y][ci]; for each cb the four rows form one 128-byte
+// segment. Optionally accumulates max |image| per tilt (for reduced-precision
+// storage) with one atomic per workgroup.
+template <int X>
+__global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
+    const float *__restrict__ img, size_t tstride, int Y, int XB,
+    float2 *__restrict__ F, uint32_t *__restrict__ tmax, int t0,
+    const float2 *__restrict__ tw) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  __shared__ float2 s[2][X];
+  __shared__ float red[16];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int t = blockIdx.y;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float *im = img + (size_t)t * tstride;
+  float2 v[E];
+  float amax = 0.0f;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    const float a = ya < Y ? im[(size_t)ya * X + n] : 0.0f;
+    const float b = yb < Y ? im[(size_t)yb * X + n] : 0.0f;
+    v[m] = make_float2(a, b);
+    amax = fmaxf(amax, fmaxf(fabsf(a), fabsf(b)));
+  }
+  Fft<X, E, -1, 1>::run(v, s[p], tid, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) s[p][tid + m * NT] = v[m];
+  __syncthreads();
+  if (tmax != nullptr) {
+    amax = block_max<2 * NT>(amax, red);
+    if (threadIdx.x == 0) atomicMax(&tmax[t0 + t], __float_as_uint(amax));
+  }
+  const int ncol = XB * CB;
+  for (int e = threadIdx.x; e < 4 * ncol; e += 2 * NT) {
+    const int ci = e % CB, r = (e / CB) % 4, cb = e / (4 * CB);
+    const int kx = cb * CB + ci, y = blockIdx.x * 4 + r;
+    if (y >= Y) continue;
+    float2 val = make_float2(0.0f, 0.0f);
+    if (kx <= X / 2) {
+      const float2 z = s[r / 2][kx];
+      const float2 zm = s[r / 2][(X - kx) & (X - 1)];
+      val = (r & 1) == 0 ? make_float2((z.x + zm.x) * 0.5f, (z.y - zm.y) * 0.5f)
+                         : make_float2((z.y + zm.y) * 0.5f, (zm.x - z.x) * 0.5f);
+    }
+    F[(((size_t)t * XB + cb) * Y + y) * CB + ci] = val;
+  }
+}
+
+// ---------------------------- pass 2: columns, flip per defocus, inverse ---
+
+// One block of CB columns (CB * Y / E threads). The forward column transform
+// stays in registers; every defocus k reuses it: multiply by the mask,
+// inverse transform, store G[t][k][cb][y][ci].
+template <int Y>
+__global__ __launch_bounds__(CB *(Y / default_e(Y))) void k_cols(
+    const float2 *__restrict__ F, const int8_t *__restrict__ mask,
+    float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw) {
+  constexpr int E = default_e(Y);
+  constexpr int NT = Y / E;
+  __shared__ float2 s[Y * CB];
+  const int ci = threadIdx.x % CB, tid = threadIdx.x / CB;
+  const int cb = blockIdx.x, t = blockIdx.y;
+  const float2 *src = F + ((size_t)t * XB + cb) * Y * CB;
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = src[(size_t)(tid + m * NT) * CB + ci];
+  Fft<Y, E, -1, CB>::run(v, s + ci, tid, tw);
+  for (int k = 0; k < ndf; ++k) {
+    const int8_t *mk = mask + ((size_t)k * XB + cb) * Y * CB;
+    float2 w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const float f = 0.5f * (float)mk[(size_t)(tid + m * NT) * CB + ci];
+      w[m] = make_float2(v[m].x * f, v[m].y * f);
+    }
+    Fft<Y, E, 1, CB>::run(w, s + ci, tid, tw);
+    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
+#pragma unroll
+    for (int m = 0; m < E; ++m) dst[(size_t)(tid + m * NT) * CB + ci] = w[m];
+  }
+}
+
+// ------------------------------------------- pass 3: rows, C2R + filter ----
+
+// Load Z[kx] = A[kx] + i B[kx] (Hermitian-extended) for rows ya, yb of one
+// (tilt, defocus) half spectrum g[cb][y][ci].
+__device__ __forceinline__ float2 load_pair_spectrum(const float2 *g, int X,
+                                                     int Y, int ya, int yb,
+                                                     int kx) {
+  const int kk = kx <= X / 2 ? kx : X - kx;
+  const int cb = kk / CB, ci = kk % CB;
+  const float2 zero = make_float2(0.0f, 0.0f);
+  const float2 A = ya < Y ? g[((size_t)cb * Y + ya) * CB + ci] : zero;
+  const float2 B = yb < Y ? g[((size_t)cb * Y + yb) * CB + ci] : zero;
+  return kx <= X / 2 ? make_float2(A.x - B.y, A.y + B.x)
+                     : make_float2(A.x + B.y, B.x - A.y);
+}
+
+__device__ __forceinline__ float inv_norm(const uint32_t *tmax, int t,
+                                          float mul) {
+  if (tmax == nullptr) return 1.0f;
+  const float n = __uint_as_float(tmax[t]) * mul;
+  return n > 0.0f ? 1.0f / n : 1.0f;
+}
+
+// Two (ya, yb) row pairs per workgroup; L = 2X threads-per-pair layout shared
+// by the X-point inverse (end radix EL/2) and the L-point filter (end radix
+// EL).
+template <int X, int STORE>
+__global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
+    k_rows_c2r_filter(const float2 *__restrict__ G, int Y, int XB, int ndf,
+                      int t0, int T, const float *__restrict__ ramp,
+                      const float *__restrict__ wscale,
+                      const uint32_t *__restrict__ tmax, float nrm_mul,
+                      typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+                      int YB, const float2 *__restrict__ tw) {
+  constexpr int L = 2 * X;
+  constexpr int EL = default_e(L);
+  constexpr int EX = EL / 2;
+  constexpr int NT = L / EL;
+  static_assert(X / EX == NT, "thread layouts must agree");
+  __shared__ float2 s[2][L];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+
+  float2 v[EX];
+#pragma unroll
+  for (int m = 0; m < EX; ++m)
+    v[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+  Fft<X, EX, 1, 1>::run(v, s[p], tid, tw);  // v = a + i b (unnormalised)
+#pragma unroll
+  for (int m = 0; m < EX; ++m) s[p][tid + m * NT] = v[m];
+  __syncthreads();
+  float2 u[EL];
+#pragma unroll
+  for (int m = 0; m < EL; ++m)
+    u[m] = m < EL / 2 ? s[p][tid + m * NT] : make_float2(0.0f, 0.0f);
+  __syncthreads();
+  Fft<L, EL, -1, 1>::run(u, s[p], tid, tw);
+#pragma unroll
+  for (int m = 0; m < EL; ++m) {
+    const int j = tid + m * NT;
+    const float r = ramp[j <= X ? j : L - j];
+    u[m] = make_float2(u[m].x * r, u[m].y * r);
+  }
+  Fft<L, EL, 1, 1>::run(u, s[p], tid, tw);
+  const float sc = wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+  using ST = StoreT<STORE>;
+#pragma unroll
+  for (int m = 0; m < EL / 2; ++m) {
+    const int n = tid + m * NT;
+    if (ya < Y) slab[slab_index(k, t, ya, n, T, NYB, X, YB)] = ST::pack(u[m].x * sc);
+    if (yb < Y) slab[slab_index(k, t, yb, n, T, NYB, X, YB)] = ST::pack(u[m].y * sc);
+  }
+}
+
+// Unfiltered complex-to-real rows for corr(): out[t][k][y][x] (float).
+template <int X>
+__global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_c2r(
+    const float2 *__restrict__ G, int Y, int XB, int ndf,
+    float *__restrict__ out, const float2 *__restrict__ tw) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  __shared__ float2 s[2][X];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int k = blockIdx.y, tl = blockIdx.z;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    v[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+  Fft<X, E, 1, 1>::run(v, s[p], tid, tw);
+  float *o = out + ((size_t)tl * ndf + k) * Y * X;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    if (ya < Y) o[(size_t)ya * X + n] = v[m].x;
+    if (yb < Y) o[(size_t)yb * X + n] = v[m].y;
+  }
+}
+
+// ----------------------------------------------- filter from real rows ----
+
+// Rows r = (k, t, y) with y fastest, read at src + t*st + k*sk + y*sy, each
+// N samples, zero-padded to L >= 2N - 1 (the ramp table accounts for L).
+// Two row pairs (four rows) per workgroup.
+template <int L, int STORE>
+__global__ __launch_bounds__(2 * (L / default_e(L))) void k_filter_rows(
+    const float *__restrict__ src, size_t st, size_t sk, size_t sy, int N,
+    int Y, int T, int ndf, const float *__restrict__ ramp,
+    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
+    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+    int YB, const float2 *__restrict__ tw) {
+  constexpr int EL = default_e(L);
+  constexpr int NT = L / EL;
+  __shared__ float2 s[2][L];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const long nrows = (long)ndf * T * Y;
+  const long r0 = ((long)blockIdx.x * 2 + p) * 2, r1 = r0 + 1;
+  int k0 = 0, t0 = 0, y0 = 0, k1 = 0, t1 = 0, y1 = 0;
+  if (r0 < nrows) {
+    y0 = (int)(r0 % Y);
+    t0 = (int)((r0 / Y) % T);
+    k0 = (int)(r0 / ((long)Y * T));
+  }
+  if (r1 < nrows) {
+    y1 = (int)(r1 % Y);
+    t1 = (int)((r1 / Y) % T);
+    k1 = (int)(r1 / ((long)Y * T));
+  }
+  const float *a = src + t0 * st + k0 * sk + y0 * sy;
+  const float *b = src + t1 * st + k1 * sk + y1 * sy;
+  float2 u[EL];
+#pragma unroll
+  for (int m = 0; m < EL; ++m) {
+    const int n = tid + m * NT;
+    const float va = (r0 < nrows && n < N) ? a[n] : 0.0f;
+    const float vb = (r1 < nrows && n < N) ? b[n] : 0.0f;
+    u[m] = make_float2(va, vb);
+  }
+  Fft<L, EL, -1, 1>::run(u, s[p], tid, tw);
+#pragma unroll
+  for (int m = 0; m < EL; ++m) {
+    const int j = tid + m * NT;
+    const float r = ramp[j <= L / 2 ? j : L - j];
+    u[m] = make_float2(u[m].x * r, u[m].y * r);
+  }
+  Fft<L, EL, 1, 1>::run(u, s[p], tid, tw);
+  using ST = StoreT<STORE>;
+  const float sa = r0 < nrows ? wscale[t0] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t0, nrm_mul)) : 0.0f;
+  const float sb = r1 < nrows ? wscale[t1] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t1, nrm_mul)) : 0.0f;
+#pragma unroll
+  for (int m = 0; m < EL; ++m) {
+    const int n = tid + m * NT;
+    if (n < N) {
+      if (r0 < nrows) slab[slab_index(k0, t0, y0, n, T, NYB, N, YB)] = ST::pack(u[m].x * sa);
+      if (r1 < nrows) slab[slab_index(k1, t1, y1, n, T, NYB, N, YB)] = ST::pack(u[m].y * sb);
+    }
+  }
+}
+
+// max |row| per tilt over rows (k, t, y), for reduced-precision scaling.
+__global__ void k_absmax_rows(const float *__restrict__ src, size_t st,
+                              size_t sk, size_t sy, int N, int Y, int T,
+                              int ndf, uint32_t *__restrict__ tmax) {
+  __shared__ float red[4];
+  const long r = blockIdx.x;
+  const int y = (int)(r % Y), t = (int)((r / Y) % T), k = (int)(r / ((long)Y * T));
+  const float *a = src + t * st + k * sk + y * sy;
+  float m = 0.0f;
+  for (int n = threadIdx.x; n < N; n += blockDim.x) m = fmaxf(m, fabsf(a[n]));
+  m = block_max<256>(m, red);
+  if (threadIdx.x == 0) atomicMax(&tmax[t], __float_as_uint(m));
+}
+
+// ------------------------------------------------------------ launchers ---
+
+#define GX_POW2_CASES(M) \
+  M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096)
+
+hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
+                           int XB, int ntilts, float2 *F, uint32_t *tmax,
+                           int t0, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  dim3 grid((Y + 3) / 4, ntilts);
+  switch (X) {
+#define C_(n)                                                                  \
+  case n:                                                                      \
+    hipLaunchKernelGGL(k_rows_r2c<n>, grid, dim3(2 * (n / default_e(n))), 0, s, \
+                       img, tstride, Y, XB, F, tmax, t0, tw);                  \
+    break;
+    GX_POW2_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
+                       int XB, int ndf, int ntilts, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  dim3 grid(XB, ntilts);
+  switch (Y) {
+#define C_(n)                                                                 \
+  case n:                                                                     \
+    hipLaunchKernelGGL(k_cols<n>, grid, dim3(CB * (n / default_e(n))), 0, s,  \
+                       F, mask, G, XB, ndf, tw);                              \
+    break;
+    GX_POW2_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
+                                      int ndf, int t0, int ntilts, int T,
+                                      const float *ramp, const float *wscale,
+                                      const uint32_t *tmax, float nrm_mul,
+                                      void *slab, int NYB, int YB,
+                                      hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  dim3 grid((Y + 3) / 4, ndf, ntilts);
+  switch (X) {
+#define C_(n)                                                                  \
+  case n:                                                                      \
+    hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE>), grid,                    \
+                       dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
+                       ndf, t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab,    \
+                       NYB, YB, tw);                                           \
+    break;
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+    C_(4096)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
+                                  int ndf, int t0, int ntilts, int T,
+                                  const float *ramp, const float *wscale,
+                                  const uint32_t *tmax, float nrm_mul,
+                                  int store, void *slab, int NYB, int YB,
+                                  hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  switch (store) {
+    case kF32:
+      return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T, ramp,
+                                       wscale, tmax, nrm_mul, slab, NYB, YB,
+                                       s, tw);
+    case kF16:
+      return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T, ramp,
+                                       wscale, tmax, nrm_mul, slab, NYB, YB,
+                                       s, tw);
+    case kBF16:
+      return launch_c2r_filter_t<kBF16>(G, X, Y, XB, ndf, t0, ntilts, T,
+                                        ramp, wscale, tmax, nrm_mul, slab,
+                                        NYB, YB, s, tw);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
+                           int ntilts, float *out, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  dim3 grid((Y + 3) / 4, ndf, ntilts);
+  switch (X) {
+#define C_(n)                                                                 \
+  case n:                                                                     \
+    hipLaunchKernelGGL(k_rows_c2r<n>, grid, dim3(2 * (n / default_e(n))), 0,  \
+                       s, G, Y, XB, ndf, out, tw);                            \
+    break;
+    GX_POW2_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_filter_rows_t(const float *src, size_t st, size_t sk,
+                                       size_t sy, int N, int L, int Y, int T,
+                                       int ndf, const float *ramp,
+                                       const float *wscale,
+                                       const uint32_t *tmax, float nrm_mul,
+                                       void *slab, int NYB, int YB,
+                                       hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  const long nrows = (long)ndf * T * Y;
+  dim3 grid((unsigned)((nrows + 3) / 4));
+  switch (L) {
+#define C_(n)                                                                 \
+  case n:                                                                     \
+    hipLaunchKernelGGL((k_filter_rows<n, STORE>), grid,                       \
+                       dim3(2 * (n / default_e(n))), 0, s, src, st, sk, sy,   \
+                       N, Y, T, ndf, ramp, wscale, tmax, nrm_mul, (ST *)slab, \
+                       NYB, YB, tw);                                          \
+    break;
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+    C_(4096) C_(8192)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
+                              size_t sy, int N, int L, int Y, int T, int ndf,
+                              const float *ramp, const float *wscale,
+                              const uint32_t *tmax, float nrm_mul, int store,
+                              void *slab, int NYB, int YB, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  switch (store) {
+    case kF32:
+      return launch_filter_rows_t<kF32>(src, st, sk, sy, N, L, Y, T, ndf,
+                                        ramp, wscale, tmax, nrm_mul, slab,
+                                        NYB, YB, s, tw);
+    case kF16:
+      return launch_filter_rows_t<kF16>(src, st, sk, sy, N, L, Y, T, ndf,
+                                        ramp, wscale, tmax, nrm_mul, slab,
+                                        NYB, YB, s, tw);
+    case kBF16:
+      return launch_filter_rows_t<kBF16>(src, st, sk, sy, N, L, Y, T, ndf,
+                                         ramp, wscale, tmax, nrm_mul, slab,
+                                         NYB, YB, s, tw);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
+                              size_t sy, int N, int Y, int T, int ndf,
+                              uint32_t *tmax, hipStream_t s) {
+  const long nrows = (long)ndf * T * Y;
+  hipLaunchKernelGGL(k_absmax_rows, dim3((unsigned)nrows), dim3(256), 0, s,
+                     src, st, sk, sy, N, Y, T, ndf, tmax);
+  return hipGetLastError();
+}
+
+}  // namespace gx

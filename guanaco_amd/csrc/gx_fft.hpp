@@ -1,0 +1,254 @@
+// LDS-resident Stockham FFT building blocks for gfx950.
+//
+// One transform of N points (N a power of two) is computed by N/E threads of
+// a workgroup, each holding E complex values in registers. The radix plan is
+// E, <middle radices>, E: the first stage reads the thread's registers
+// (x[tid + m*N/E]) and the last stage leaves X[tid + m*N/E] in registers, so
+// producers and consumers (global loads, the CTF flip, the ramp multiply,
+// stores) fuse directly into the first and last stages. Middle stages
+// exchange through LDS. Several transforms can share a workgroup; each uses
+// its own LDS region, addressed as lds[i * LS] so that columns can be
+// interleaved. Every thread of the workgroup must call run() (it contains
+// __syncthreads()).
+//
+// Conventions follow FFTW (the reference's host FFT, src/libguanaco/
+// fft.cpp:38-71): DIR = -1 is exp(-2 pi i jk/N), DIR = +1 is exp(+...), both
+// unnormalised. Twiddles come from a table W[j] = exp(-2 pi i j / kTwN)
+// computed in double on the host and rounded to float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gx {
+
+constexpr int kTwN = 16384;  // twiddle table length: largest supported N
+
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) {
+  return make_float2(a.x + b.x, a.y + b.y);
+}
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) {
+  return make_float2(a.x - b.x, a.y - b.y);
+}
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by DIR * i
+template <int DIR>
+__host__ __device__ __forceinline__ float2 mul_i(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+template <int DIR>
+__device__ __forceinline__ float2 twiddle(const float2 *__restrict__ tw,
+                                          int idx) {
+  float2 w = tw[idx];
+  return DIR < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// ---- radix kernels: v[k] <- sum_n v[n] exp(DIR 2 pi i nk / R) -------------
+
+template <int DIR>
+__host__ __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2,
+                                     float2 &a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_i<DIR>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// a * exp(DIR i pi j / 8) for the constant j (0 <= j < 16)
+template <int DIR, int J>
+__host__ __device__ __forceinline__ float2 rot16(float2 a) {
+  constexpr int j = J & 15;
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (j == 4) {
+    return mul_i<DIR>(a);
+  } else if constexpr (j == 8) {
+    return make_float2(-a.x, -a.y);
+  } else if constexpr (j == 12) {
+    return mul_i<-DIR>(a);
+  } else {
+    constexpr float C1 = 0.92387953251128674f;  // cos(pi/8)
+    constexpr float S1 = 0.38268343236508978f;  // sin(pi/8)
+    constexpr float R2 = 0.70710678118654752f;  // cos(pi/4)
+    constexpr float ct[16] = {1, C1, R2, S1, 0, -S1, -R2, -C1,
+                              -1, -C1, -R2, -S1, 0, S1, R2, C1};
+    constexpr float st[16] = {0, S1, R2, C1, 1, C1, R2, S1,
+                              0, -S1, -R2, -C1, -1, -C1, -R2, -S1};
+    return cmul(a, make_float2(ct[j], DIR * st[j]));
+  }
+}
+
+template <int R, int DIR>
+__host__ __device__ __forceinline__ void dft(float2 *v) {
+  if constexpr (R == 1) {
+    return;
+  } else if constexpr (R == 2) {
+    float2 t = v[0];
+    v[0] = cadd(t, v[1]);
+    v[1] = csub(t, v[1]);
+  } else if constexpr (R == 4) {
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    o1 = rot16<DIR, 2>(o1);
+    o2 = rot16<DIR, 4>(o2);
+    o3 = rot16<DIR, 6>(o3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+  } else if constexpr (R == 16) {
+    // 4 x 4: X[k1 + 4 k2] = sum_n2 W4^{n2 k2} W16^{n2 k1} DFT4_n1(v[n2+4n1])
+    float2 b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      float2 a0 = v[n2], a1 = v[n2 + 4], a2 = v[n2 + 8], a3 = v[n2 + 12];
+      dft4<DIR>(a0, a1, a2, a3);
+      b[n2 * 4 + 0] = a0;
+      b[n2 * 4 + 1] = a1;
+      b[n2 * 4 + 2] = a2;
+      b[n2 * 4 + 3] = a3;
+    }
+    // twiddle W16^{n2 k1}
+    b[5] = rot16<DIR, 1>(b[5]);
+    b[6] = rot16<DIR, 2>(b[6]);
+    b[7] = rot16<DIR, 3>(b[7]);
+    b[9] = rot16<DIR, 2>(b[9]);
+    b[10] = rot16<DIR, 4>(b[10]);
+    b[11] = rot16<DIR, 6>(b[11]);
+    b[13] = rot16<DIR, 3>(b[13]);
+    b[14] = rot16<DIR, 6>(b[14]);
+    b[15] = rot16<DIR, 9>(b[15]);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 a0 = b[k1], a1 = b[4 + k1], a2 = b[8 + k1], a3 = b[12 + k1];
+      dft4<DIR>(a0, a1, a2, a3);
+      v[k1] = a0;
+      v[k1 + 4] = a1;
+      v[k1 + 8] = a2;
+      v[k1 + 12] = a3;
+    }
+  }
+}
+
+// ---- radix plan -----------------------------------------------------------
+
+__host__ __device__ constexpr int ilog2c(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// Plan for N with end radix E: E, middle..., E where the middle product
+// N / E^2 is split into radices <= E, largest first. N == E is one stage.
+__host__ __device__ constexpr int plan_stages(int N, int E) {
+  if (N <= E) return 1;
+  int mid = N / (E * E), s = 2;
+  while (mid > 1) {
+    int r = mid >= E ? E : mid;
+    mid /= r;
+    ++s;
+  }
+  return s;
+}
+__host__ __device__ constexpr int plan_radix(int N, int E, int stage) {
+  int ns = plan_stages(N, E);
+  if (ns == 1) return N;
+  if (stage == 0 || stage == ns - 1) return E;
+  int mid = N / (E * E);
+  for (int s = 1; s < ns - 1; ++s) {
+    int r = mid >= E ? E : mid;
+    if (s == stage) return r;
+    mid /= r;
+  }
+  return 1;
+}
+// product of the radices of the stages before `stage`
+__host__ __device__ constexpr int plan_ns(int N, int E, int stage) {
+  int p = 1;
+  for (int s = 0; s < stage; ++s) p *= plan_radix(N, E, s);
+  return p;
+}
+// Default end radix: 16 for N >= 256, else the largest E with E^2 <= N
+// (single stage up to 16 points when N is itself a valid radix).
+__host__ __device__ constexpr int default_e(int N) {
+  return N >= 256 ? 16 : (N >= 64 ? 8 : (N >= 16 ? 4 : (N >= 4 ? 2 : N)));
+}
+
+template <int N, int E, int DIR, int LS>
+struct Fft {
+  static_assert((N & (N - 1)) == 0, "N must be a power of two");
+  static_assert(N <= kTwN, "N exceeds the twiddle table");
+  static constexpr int NT = N / E;  // threads per transform
+  static constexpr int NS = plan_stages(N, E);
+  static_assert(NS == 1 || N >= E * E, "invalid end radix");
+
+  template <int S>
+  __device__ __forceinline__ static void middle(float2 *lds, int tid,
+                                                const float2 *tw) {
+    if constexpr (S < NS - 1) {
+      constexpr int R = plan_radix(N, E, S);
+      constexpr int Ns = plan_ns(N, E, S);
+      constexpr int NB = N / R;
+      constexpr int Q = E / R;
+      constexpr int TWS = kTwN / (Ns * R);
+      float2 w[E];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int b = tid + q * NT;
+#pragma unroll
+        for (int m = 0; m < R; ++m) w[q * R + m] = lds[(b + m * NB) * LS];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int b = tid + q * NT;
+        const int k = b & (Ns - 1);
+#pragma unroll
+        for (int m = 1; m < R; ++m)
+          w[q * R + m] = cmul(w[q * R + m], twiddle<DIR>(tw, m * k * TWS));
+        dft<R, DIR>(&w[q * R]);
+        const int base = (b - k) * R + k;
+#pragma unroll
+        for (int m = 0; m < R; ++m) lds[(base + m * Ns) * LS] = w[q * R + m];
+      }
+      __syncthreads();
+      middle<S + 1>(lds, tid, tw);
+    }
+  }
+
+  // In: v[m] = x[tid + m*NT]. Out: v[m] = X[tid + m*NT]. lds holds N*LS
+  // float2 (only every LS-th used). Returns with lds free for reuse.
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds,
+                                             int tid, const float2 *tw) {
+    if constexpr (NS == 1) {
+      dft<E, DIR>(v);
+    } else {
+      dft<E, DIR>(v);
+#pragma unroll
+      for (int m = 0; m < E; ++m) lds[(tid * E + m) * LS] = v[m];
+      __syncthreads();
+      middle<1>(lds, tid, tw);
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[m] = lds[(tid + m * NT) * LS];
+      __syncthreads();
+      constexpr int TWS = kTwN / N;
+#pragma unroll
+      for (int m = 1; m < E; ++m)
+        v[m] = cmul(v[m], twiddle<DIR>(tw, m * tid * TWS));
+      dft<E, DIR>(v);
+    }
+  }
+};
+
+}  // namespace gx

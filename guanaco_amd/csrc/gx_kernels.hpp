@@ -1,0 +1,59 @@
+// Kernel launchers shared by the C-ABI layer (host side, compiled by hipcc).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gx {
+
+struct BpArgs {
+  const void *slab;          // [k][t][y/YB][x][y%YB]
+  const float4 *coef;        // per tilt: c, s, offset, scale (reference math)
+  const uint32_t *tmax;      // per tilt normalisation bits (reduced storage)
+  float nrm_mul;             // normalisation = float(tmax[t]) * nrm_mul
+  float dscale, doffset;     // slab selection (reconstructor.cpp:29-32)
+  int ndf, N, T, NYB, W, Z, Y;
+  float *vol;                // [y][z][x]
+};
+
+// Each returns hipSuccess or the launch error.
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+
+hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
+                           const double *d_defocus, double l, double Cs,
+                           double Ca, double Pa, double phi, double ps,
+                           hipStream_t s);
+hipError_t launch_mask_from_ctf(int8_t *mask, const float2 *ctf, int X, int Y,
+                                int XB, int ndf, hipStream_t s);
+
+hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
+                           int XB, int ntilts, float2 *F, uint32_t *tmax,
+                           int t0, hipStream_t s);
+hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
+                       int XB, int ndf, int ntilts, hipStream_t s);
+hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
+                                  int ndf, int t0, int ntilts, int T,
+                                  const float *ramp, const float *wscale,
+                                  const uint32_t *tmax, float nrm_mul,
+                                  int store, void *slab, int NYB, int YB,
+                                  hipStream_t s);
+hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
+                           int ntilts, float *out, hipStream_t s);
+hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
+                              size_t sy, int N, int L, int Y, int T, int ndf,
+                              const float *ramp, const float *wscale,
+                              const uint32_t *tmax, float nrm_mul, int store,
+                              void *slab, int NYB, int YB, hipStream_t s);
+hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
+                              size_t sy, int N, int Y, int T, int ndf,
+                              uint32_t *tmax, hipStream_t s);
+
+// store: kF32 (exact reference arithmetic) or kF16/kBF16 (FP32 accumulate).
+hipError_t launch_bp(const BpArgs &a, int YB, int store, hipStream_t s);
+
+// Twiddle table W[j] = exp(-2 pi i j / kTwN) for the current device.
+const float2 *twiddles(hipError_t *err);
+
+}  // namespace gx

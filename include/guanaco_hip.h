@@ -1,0 +1,180 @@
+/*
+ * guanaco_hip.h - C ABI of the MI355X (gfx950) reconstruction library
+ * libguanaco_hip.so.
+ *
+ * Plain C: pointers, sizes and status codes only. Every entry point returns
+ * GX_OK (0) or a non-zero GX_E* code; gx_last_error() then describes the
+ * failure (thread-local). All entry points are thread-safe; none uses device
+ * globals, so calls on different streams may overlap.
+ *
+ * The reference's GPU path is reached through pybind11 (guanaco_ext.recon /
+ * guanaco_ext.corr) and C++ classes; the entries below replace those classes
+ * one for one (reference file:line given with each), and add a
+ * device-resident whole-volume path that the reference does not have.
+ */
+#ifndef GUANACO_HIP_H
+#define GUANACO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GX_OK 0
+#define GX_EINVAL 1       /* invalid argument or configuration */
+#define GX_ENOMEM 2       /* device allocation failed */
+#define GX_EDEVICE 3      /* HIP runtime / kernel error */
+#define GX_EUNSUPPORTED 4 /* size not supported by the gfx950 kernels */
+
+/* Storage precision of filtered projections (accumulation is always FP32). */
+#define GX_STORE_F32 0  /* exact reference FP32 arithmetic */
+#define GX_STORE_F16 1
+#define GX_STORE_BF16 2
+
+/* ---- status / devices ---------------------------------------------------- */
+
+/* Message of the last failure on this thread (GUANACO_ASSERT's text in the
+ * reference, include/guanaco/error.h:31-61). */
+const char *gx_last_error(void);
+int gx_version(void);
+int gx_device_count(int *count);
+int gx_set_device(int device);
+int gx_get_device(int *device);
+/* name (NUL-terminated, truncated to name_len), total HBM bytes, CU count */
+int gx_device_info(int device, char *name, size_t name_len, size_t *total_mem,
+                   int *num_cus);
+int gx_mem_get_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- thin device memory / stream layer ----------------------------------- */
+/* Streams and events are opaque HIP handles (NULL = the default stream). */
+
+int gx_malloc(void **ptr, size_t bytes);
+int gx_free(void *ptr);
+int gx_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int gx_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int gx_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+/* Fill with a 32-bit pattern by kernel (safe under stream capture). */
+int gx_fill_u32(void *dst, uint32_t value, size_t count, void *stream);
+int gx_stream_create(void **stream);
+int gx_stream_destroy(void *stream);
+int gx_stream_synchronize(void *stream);
+int gx_device_synchronize(void);
+int gx_event_create(void **event);
+int gx_event_destroy(void *event);
+int gx_event_record(void *event, void *stream);
+int gx_event_synchronize(void *event);
+int gx_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Stream capture into a graph and replay (hipStreamBeginCapture /
+ * hipStreamEndCapture / hipGraphInstantiate / hipGraphLaunch). */
+int gx_graph_capture_begin(void *stream);
+int gx_graph_capture_end(void *stream, void **graph_exec);
+int gx_graph_launch(void *graph_exec, void *stream);
+int gx_graph_destroy(void *graph_exec);
+
+/* ---- reference-equivalent entry points (host buffers, synchronous) -------- */
+
+/* Mirrors guanaco::Config (include/guanaco/config.h:29-76). */
+typedef struct gx_recon_cfg {
+  size_t num_pixels;   /* N, detector columns */
+  size_t num_angles;   /* T */
+  size_t num_defocus;  /* ndf */
+  size_t grid_height;  /* H (depth) */
+  size_t grid_width;   /* W */
+  float pixel_size;    /* A */
+  float centre;        /* rotation centre, pixels */
+  float min_defocus;   /* A, relative to the defocus at the tilt axis */
+  float max_defocus;
+  const float *angles; /* host, num_angles radians */
+  int gpu_index;       /* device to use; -1 = current */
+} gx_recon_cfg;
+
+/* Per-tilt constants exactly as reconstructor.cpp:36-47 computes them:
+ * coef[4t..4t+3] = {cos/d, sin/d, offset, scale} (float). Host only. */
+int gx_tilt_geometry(const gx_recon_cfg *cfg, float *coef);
+
+/* Replaces Reconstructor_t<e_device>::operator() (reconstructor.cu:317-354)
+ * with the CPU path's semantics (reconstructor.cpp:92-115): filters a copy
+ * of sino (num_defocus x num_angles x num_pixels, host), overwrites rec
+ * (grid_height x grid_width, host) with the backprojection. */
+int gx_recon(const gx_recon_cfg *cfg, const float *sino, float *rec);
+
+/* Backprojection only (reconstructor.cpp:14-90) of an already filtered
+ * sinogram; rec is overwritten (the reference zeroes it first). */
+int gx_project(const gx_recon_cfg *cfg, const float *filtered, float *rec);
+
+/* Ramp filter only (filter.cpp:61-97 / filter.cu:42-100): every row of
+ * rows (nrows x num_pixels, host) filtered into out. */
+int gx_filter(const float *rows, float *out, size_t nrows, size_t num_pixels,
+              int gpu_index);
+
+/* Replaces Corrector<e_device, float>::correct (correct.cu:30-86) behind
+ * guanaco::correct (correct.h:77-95): image (ysize x xsize), ctf complex64
+ * interleaved (num_ctf x ysize x xsize), rec (num_ctf x ysize x xsize)
+ * overwritten with Re(IFFT2_unnorm(FFT2(image) * -sign(Im ctf))).
+ * xsize and ysize must be powers of two in [8, 4096] and [4, 4096]. */
+int gx_corr(const float *image, const float *ctf, float *rec, size_t xsize,
+            size_t ysize, size_t num_ctf, int gpu_index);
+
+/* ---- device-resident whole-volume path ------------------------------------ */
+
+/* Input layouts for gx_volume_run. */
+#define GX_INPUT_PROJECTIONS 0 /* (T, Y, X) raw projections */
+#define GX_INPUT_CORRECTED 1   /* (T, ndf, Y, X) already CTF-corrected */
+#define GX_INPUT_SINOGRAMS 2   /* (Y, ndf, T, X) sinogram order */
+
+typedef struct gx_volume_desc {
+  size_t num_pixels;   /* X = N */
+  size_t num_rows;     /* Y, detector rows = volume slices */
+  size_t num_angles;   /* T */
+  size_t num_defocus;  /* ndf */
+  size_t depth;        /* Z, grid height (central rows of the square grid) */
+  size_t width;        /* W, grid width */
+  float pixel_size;    /* A */
+  float centre;        /* rotation centre, pixels (same for every row) */
+  float min_defocus;   /* A, relative, as reconstruct() passes to recon() */
+  float max_defocus;
+  const float *angles;  /* host, T radians */
+  const float *weights; /* host, T per-tilt multipliers, or NULL for ones */
+  int storage;          /* GX_STORE_* */
+  int input_layout;     /* GX_INPUT_* */
+  /* On-device CTF correction of GX_INPUT_PROJECTIONS: when defocus is not
+   * NULL, slab k is phase-flipped with the absolute defocus defocus[k] (A),
+   * as correct_projections does (correct.py:217-318). */
+  const double *defocus;
+  double wavelength; /* A */
+  double Cs;         /* A */
+  double Ca;         /* A */
+  double Pa;         /* rad */
+  double phi;        /* rad */
+  size_t tilt_chunk; /* tilts per correction launch; 0 = automatic */
+  int gpu_index;     /* -1 = current device */
+} gx_volume_desc;
+
+typedef struct gx_volume_plan gx_volume_plan;
+
+/* Device bytes the plan holds (filtered slabs + workspace), excluding the
+ * caller's input and volume: the admission footprint. */
+int gx_volume_plan_bytes(const gx_volume_desc *desc, size_t *bytes);
+int gx_volume_plan_create(const gx_volume_desc *desc, gx_volume_plan **plan);
+int gx_volume_plan_destroy(gx_volume_plan *plan);
+/* Enqueue the whole path on `stream`: d_in (layout per input_layout, device)
+ * -> d_vol (Y x Z x W float32, device, overwritten). No allocation, no
+ * memset: the call can be captured into a graph. Calls that share a plan
+ * must not overlap. */
+int gx_volume_run(gx_volume_plan *plan, const float *d_in, float *d_vol,
+                  void *stream);
+/* Stages of gx_volume_run, for timing: filter/correct into the plan's slabs,
+ * and backprojection of the slabs. */
+int gx_volume_filter(gx_volume_plan *plan, const float *d_in, void *stream);
+int gx_volume_backproject(gx_volume_plan *plan, float *d_vol, void *stream);
+/* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage. */
+int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
+                    int *rows_per_block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -137,6 +137,17 @@ static tilt_coef tilt_coefficients(const or_config *cfg, size_t angle) {
   return tc;
 }
 
+void or_tilt_coefficients(const or_config *cfg, float *coef) {
+  size_t t;
+  for (t = 0; t < cfg->num_angles; ++t) {
+    tilt_coef tc = tilt_coefficients(cfg, t);
+    coef[4 * t + 0] = tc.c;
+    coef[4 * t + 1] = tc.s;
+    coef[4 * t + 2] = tc.offset;
+    coef[4 * t + 3] = tc.scale;
+  }
+}
+
 /* reconstructor.cpp:29-32: dscale = ndf * ps / (max - min) (float),
  * doffset = -dscale * (min / ps). */
 static void defocus_coefficients(const or_config *cfg, float *dscale,

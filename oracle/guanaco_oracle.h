@@ -90,6 +90,9 @@ typedef struct {
 /* config.h:66-75 */
 int or_config_is_valid(const or_config *cfg);
 
+/* reconstructor.cpp:36-47: per-tilt {cos/d, sin/d, offset, scale}. */
+void or_tilt_coefficients(const or_config *cfg, float *coef);
+
 /* reconstructor.cpp:14-90: rec (H x W) += BP of the filtered sinogram
  * (ndf x T x N). rec is NOT cleared here, matching project(). */
 void or_project_f32(const or_config *cfg, const float *sino, float *rec);

@@ -83,6 +83,7 @@ def lib():
         L.or_recon_f64.argtypes = [ctypes.POINTER(OrConfig), _f64p, _f64p,
                                    ctypes.c_int]
         L.or_config_is_valid.argtypes = [ctypes.POINTER(OrConfig)]
+        L.or_tilt_coefficients.argtypes = [ctypes.POINTER(OrConfig), _f32p]
         _LIB = L
     return _LIB
 
@@ -236,6 +237,15 @@ def _sino_cfg(sino, grid_shape, angles, centre, pixel_size, min_defocus,
     H, W = grid_shape
     return make_config(N, T, ndf, H, W, pixel_size, centre, min_defocus,
                        max_defocus, angles)
+
+
+def tilt_coefficients(angles, centre):
+    """reconstructor.cpp:36-47: (T, 4) float32 {cos/d, sin/d, offset, scale}."""
+    angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
+    cfg = make_config(1, angles.size, 1, 1, 1, 1.0, centre, 0.0, 0.0, angles)
+    out = numpy.empty((angles.size, 4), numpy.float32)
+    lib().or_tilt_coefficients(ctypes.byref(cfg), _p32(out))
+    return out
 
 
 def project(sino, grid_shape, angles, centre, pixel_size=1.0, min_defocus=0.0,

@@ -1,0 +1,214 @@
+"""GPU parity: every kernel path through the C ABI against the CPU oracle.
+
+Tolerances: the FP32 path within 1e-5 relative L2 of the FP32 oracle
+(backprojection of an identical filtered sinogram bit-exact); the FP16/BF16
+storage forms within the stated bound of the FP64 oracle.
+"""
+import ctypes
+import math
+
+import numpy
+import pytest
+
+from guanaco_amd import _capi as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = numpy.asarray(a, numpy.float64)
+    b = numpy.asarray(b, numpy.float64)
+    return float(numpy.linalg.norm(a - b) / max(numpy.linalg.norm(b), 1e-300))
+
+
+def tilts(T, span=60.0):
+    if T == 1:
+        return numpy.zeros(1, numpy.float32)
+    return numpy.deg2rad(numpy.linspace(-span, span, T)).astype(numpy.float32)
+
+
+# ------------------------------------------------------------- filter ------
+
+@pytest.mark.parametrize("N", [8, 13, 64, 100, 256, 1024, 1500, 4096])
+def test_filter_matches_oracle(capi, N):
+    rng = numpy.random.default_rng(N)
+    rows = rng.standard_normal((5, N)).astype(numpy.float32)
+    out = numpy.empty_like(rows)
+    C.call("gx_filter", C.f32p(rows), C.f32p(out), 5, N, -1)
+    expect = O.filter_rows(rows, "f32")
+    assert rel_l2(out, expect) < 1e-6, N
+
+
+# ----------------------------------------------------- backprojection -----
+
+BP_CASES = [
+    # N, T, ndf, H, W, centre, (min_df, max_df), pixel
+    (16, 1, 1, 16, 16, 8.0, (0.0, 0.0), 1.0),
+    (64, 7, 1, 64, 64, 32.0, (0.0, 0.0), 1.0),
+    (64, 61, 1, 16, 64, 31.3, (0.0, 0.0), 1.0),
+    (100, 9, 3, 100, 100, 47.5, (-150.0, 150.0), 1.5),
+    (64, 33, 5, 32, 80, 30.0, (-64.0, 64.0), 1.0),
+    (128, 17, 16, 128, 128, 64.0, (-96.0, 96.0), 1.5),
+]
+
+
+@pytest.mark.parametrize("case", BP_CASES)
+def test_project_bitexact_vs_oracle(capi, case):
+    N, T, ndf, H, W, centre, (mn, mx), ps = case
+    rng = numpy.random.default_rng(N * T + ndf)
+    angles = tilts(T)
+    sino = rng.standard_normal((ndf, T, N)).astype(numpy.float32)
+    cfg = C.recon_cfg(N, T, ndf, H, W, ps, centre, mn, mx, angles)
+    got = numpy.empty((H, W), numpy.float32)
+    C.call("gx_project", ctypes.byref(cfg), C.f32p(sino), C.f32p(got))
+    expect = O.project(sino if ndf > 1 else sino[0], (H, W), angles, centre,
+                       ps, mn, mx)
+    numpy.testing.assert_array_equal(got, expect)
+
+
+@pytest.mark.parametrize("case", BP_CASES[1:4])
+def test_recon_matches_oracle(capi, case):
+    N, T, ndf, H, W, centre, (mn, mx), ps = case
+    rng = numpy.random.default_rng(7 + N)
+    angles = tilts(T)
+    sino = rng.standard_normal((ndf, T, N)).astype(numpy.float32)
+    cfg = C.recon_cfg(N, T, ndf, H, W, ps, centre, mn, mx, angles)
+    got = numpy.empty((H, W), numpy.float32)
+    C.call("gx_recon", ctypes.byref(cfg), C.f32p(sino), C.f32p(got))
+    expect = O.recon(sino, (H, W), angles, centre, ps, mn, mx)
+    assert rel_l2(got, expect) < 1e-5
+
+
+# ---------------------------------------------------------- correction ----
+
+def ctf_stack(X, Y, ps, dfs, Cs=2.7e7, Ca=0.0, Pa=0.0, phi=0.0):
+    l = O.electron_wavelength(300000)
+    return numpy.stack([O.ctf_simple_grid(l, df, Cs, Ca, Pa, phi, X, Y, ps)
+                        for df in dfs]).astype(numpy.complex64)
+
+
+CORR_CASES = [
+    # Y, X, ps, dfs, Ca, Pa, phi
+    (16, 16, 2.0, [20000.0], 0.0, 0.0, 0.0),
+    (32, 64, 2.0, [15000.0, 20000.0, 25000.0], 0.0, 0.0, 0.1),
+    (64, 32, 1.5, [30000.0, 31000.0], 400.0, 0.4, 0.1),
+    (256, 128, 1.0, [25000.0], 0.0, 0.0, math.asin(0.1)),
+    (1024, 1024, 2.0, [20000.0], 0.0, 0.0, math.asin(0.1)),
+]
+
+
+@pytest.mark.parametrize("case", CORR_CASES)
+def test_corr_matches_oracle(capi, case):
+    Y, X, ps, dfs, Ca, Pa, phi = case
+    rng = numpy.random.default_rng(X + Y)
+    img = rng.standard_normal((Y, X)).astype(numpy.float32) + 2.0
+    ctf = ctf_stack(X, Y, ps, dfs, Ca=Ca, Pa=Pa, phi=phi)
+    out = numpy.empty((len(dfs), Y, X), numpy.float32)
+    C.call("gx_corr", C.f32p(img), ctf.view(numpy.float32).ctypes.data_as(
+        C._f32p), C.f32p(out), X, Y, len(dfs), -1)
+    expect = O.corr(img, ctf, precision="f32")
+    for k in range(len(dfs)):
+        assert rel_l2(out[k], expect[k]) < 1e-5, (case, k)
+
+
+def test_corr_special_case_exact_negation(capi):
+    # df = Cs = 0, phi = 90 deg: out = -(X Y) image
+    Y, X = 32, 64
+    img = numpy.random.default_rng(1).standard_normal((Y, X)).astype(
+        numpy.float32)
+    ctf = ctf_stack(X, Y, 1.0, [0.0], Cs=0.0, phi=math.pi / 2)
+    out = numpy.empty((1, Y, X), numpy.float32)
+    C.call("gx_corr", C.f32p(img), ctf.view(numpy.float32).ctypes.data_as(
+        C._f32p), C.f32p(out), X, Y, 1, -1)
+    assert rel_l2(out[0], -X * Y * img.astype(numpy.float64)) < 1e-6
+
+
+# ------------------------------------------------------ whole volumes -----
+
+def ctf_problem(X, Y, T, ndf, ps, df, seed, phi=0.0, Ca=0.0, Pa=0.0,
+                step=None):
+    rng = numpy.random.default_rng(seed)
+    proj = (rng.standard_normal((T, Y, X)) + 5.0).astype(numpy.float32)
+    angles = tilts(T)
+    centre = numpy.full(Y, X / 2.0, numpy.float32)
+    mn, mx, st, n = O.get_defocus_steps(df, ndf, step, X, centre, ps)
+    dfs = numpy.array([mn + d * st for d in range(n)], dtype=numpy.float64)
+    return proj, angles, centre, dfs, (mn - df, mx - df)
+
+
+def gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, storage, dfs=None,
+               layout=C.INPUT_PROJECTIONS, phi=0.0, Ca=0.0, Pa=0.0,
+               weights=None, tilt_chunk=0):
+    l = O.electron_wavelength(300000)
+    plan = C.VolumePlan(X, Y, angles, Z, num_defocus=ndf, pixel_size=ps,
+                        centre=X / 2.0, min_defocus=rel[0],
+                        max_defocus=rel[1], storage=storage,
+                        input_layout=layout, defocus=dfs, wavelength=l,
+                        Cs=2.7e7, Ca=Ca, Pa=Pa, phi=phi, weights=weights,
+                        tilt_chunk=tilt_chunk)
+    d_in = C.DeviceArray.from_host(proj)
+    d_vol = C.DeviceArray((Y, Z, X))
+    plan.run(d_in, d_vol)
+    C.call("gx_device_synchronize")
+    out = d_vol.to_host()
+    plan.destroy()
+    return out
+
+
+VOL_CASES = [
+    # X, Y, T, ndf, Z, ps, df, chunk
+    (64, 32, 9, 1, 16, 2.0, 20000.0, 0),
+    (64, 16, 9, 3, 32, 1.5, 30000.0, 4),
+    (128, 8, 13, 8, 32, 1.0, 25000.0, 0),
+]
+
+
+@pytest.mark.parametrize("case", VOL_CASES)
+def test_volume_ctf_f32_matches_oracle_chain(capi, case):
+    X, Y, T, ndf, Z, ps, df, chunk = case
+    proj, angles, centre, dfs, rel = ctf_problem(X, Y, T, ndf, ps, df, X + T)
+    got = gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, "f32", dfs,
+                     tilt_chunk=chunk)
+    corrected, mn, mx = O.correct_projections(
+        proj, centre, ps, 300, df, ndf, None, 2.7, 0, 0, 0)
+    expect = O.reconstruct(corrected, angles, X / 2.0, ps, mn, mx, depth=Z)
+    assert rel_l2(got, expect) < 1e-5
+
+
+@pytest.mark.parametrize("storage,bound", [("f16", 1e-3), ("bf16", 4e-3)])
+def test_volume_ctf_reduced_vs_fp64_oracle(capi, storage, bound):
+    X, Y, T, ndf, Z, ps, df = 128, 16, 13, 4, 32, 1.0, 25000.0
+    proj, angles, centre, dfs, rel = ctf_problem(X, Y, T, ndf, ps, df, 99)
+    got = gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, storage, dfs)
+    corrected, mn, mx = O.correct_projections(
+        proj, centre, ps, 300, df, ndf, None, 2.7, 0, 0, 0, precision="f64")
+    expect = O.reconstruct(corrected, angles, X / 2.0, ps, mn, mx, depth=Z,
+                           precision="f64")
+    err = rel_l2(got, expect)
+    print("rel-L2 %s vs FP64 oracle: %.3g" % (storage, err))
+    assert err < bound
+
+
+def test_volume_plain_fbp_matches_oracle(capi):
+    X, Y, T, Z = 100, 12, 11, 40
+    rng = numpy.random.default_rng(3)
+    proj = rng.standard_normal((T, Y, X)).astype(numpy.float32)
+    angles = tilts(T)
+    got = gpu_volume(proj, angles, X, Y, Z, 1, 1.0, (0.0, 0.0), "f32")
+    expect = O.reconstruct(proj, angles, X / 2.0, 1.0, depth=Z)
+    assert rel_l2(got, expect) < 1e-5
+
+
+def test_volume_sinogram_layout_and_weights(capi):
+    X, Y, T, ndf, Z = 64, 10, 9, 2, 20
+    rng = numpy.random.default_rng(4)
+    sino = rng.standard_normal((Y, ndf, T, X)).astype(numpy.float32)
+    angles = (tilts(T) + rng.uniform(-0.01, 0.01, T)).astype(numpy.float32)
+    w = O.compute_angular_weights(angles)
+    got = gpu_volume(sino, angles, X, Y, Z, ndf, 1.0, (-40.0, 40.0), "f32",
+                     layout=C.INPUT_SINOGRAMS, weights=-w)
+    expect = O.reconstruct(sino, angles, X / 2.0, 1.0, -40.0, 40.0,
+                           sinogram_order=True, transform="minus",
+                           angular_weights=True, depth=Z)
+    assert rel_l2(got, expect) < 1e-5
