@@ -1,0 +1,438 @@
+"""Benchmark of the reconstruction hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+
+One step reconstructs one whole tilt series that is already resident in HBM:
+CTF phase-flip correction (2D R2C, per-defocus flip, C2R), ramp filter and
+backprojection of every defocus slab into the volume. Two paths run on the
+same input:
+
+- plain:   FP32 storage, exact reference arithmetic, kernels launched eagerly;
+- batched: FP16 storage of the filtered slabs, FP32 accumulation, the whole
+           request captured in one HIP graph and replayed.
+
+The value is the batched path's whole-job throughput in voxel*tilt updates/s
+(updates = X * Y * Z * T; ndf does not multiply it). rel-L2 of both volumes
+is measured against the FP64 CPU oracle on sampled y-slices. The roofline
+entry is for the backprojection kernel of the batched path, timed with HIP
+events on the stream it is launched on; algorithmic bytes per update are
+2 * s + 4 / T (two interpolation taps at storage size s, the FP32 voxel
+written once per voxel). The CPU baseline is the oracle (a restatement of the
+reference's CPU path) on one host core over a bounded sample, extrapolated.
+"""
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("voxel·tilt updates/s at ≤1e-3 rel-L2 vs FP64, batched vs "
+          "plain; p50/p99 latency")
+UNIT = "voxel·tilt updates/s"
+HBM_PEAK_GBS = 8000.0
+
+# name: (X, Y, T, tilt step deg, Z, ndf, pixel A, defocus A, phi rad,
+#        ellipsoids, semi-axes px, noise, seed, description)
+WORKLOADS = {
+    "c2": dict(X=1024, Y=1024, T=121, step=1.0, Z=256, ndf=1, ps=2.0,
+               df=20000.0, phi=math.asin(0.1), ne=200, semi=(20.0, 160.0),
+               dose=50.0, seed=1001,
+               name="single-defocus CTF-corrected FBP: 121 x 1024^2 "
+                    "projections (+-60 deg, 1 deg), 2 A/px, 300 kV, "
+                    "Cs 2.7 mm, df 20000 A, amplitude contrast 0.1 -> "
+                    "1024 x 256 x 1024 volume"),
+    "c3": dict(X=2048, Y=2048, T=61, step=2.0, Z=512, ndf=16, ps=1.5,
+               df=30000.0, phi=math.asin(0.1), ne=400, semi=(40.0, 320.0),
+               dose=30.0, seed=1002,
+               name="defocus-range CTF correction, ndf 16: 61 x 2048^2 "
+                    "projections -> 2048 x 512 x 2048 volume"),
+    "c1": dict(X=256, Y=256, T=61, step=2.0, Z=128, ndf=1, ps=1.0,
+               df=None, phi=0.0, ne=20, semi=(5.0, 40.0), dose=None,
+               seed=1000,
+               name="plain FBP, no CTF: 61 x 256^2 projections -> "
+                    "256 x 128 x 256 volume"),
+}
+
+
+def percentile(a, q):
+    a = sorted(a)
+    if not a:
+        return None
+    k = (len(a) - 1) * q / 100.0
+    f, c = math.floor(k), math.ceil(k)
+    return a[f] if f == c else a[f] + (a[c] - a[f]) * (k - f)
+
+
+# ---------------------------------------------------------------- GPU -----
+
+def make_plans(C, w, angles, storage):
+    from oracle import oracle as O  # only for the wavelength constant
+    l = O.electron_wavelength(300000)
+    kw = dict(num_defocus=w["ndf"], pixel_size=w["ps"], centre=w["X"] / 2.0,
+              storage=storage)
+    if w["df"] is not None:
+        centre = numpy.full(w["Y"], w["X"] / 2.0, numpy.float32)
+        mn, mx, st, n = _defocus_steps(w["df"], w["ndf"], w["X"], centre,
+                                       w["ps"])
+        dfs = numpy.array([mn + d * st for d in range(n)], numpy.float64)
+        kw.update(min_defocus=mn - w["df"], max_defocus=mx - w["df"],
+                  defocus=dfs, wavelength=l, Cs=2.7e7, phi=w["phi"])
+    return C.VolumePlan(w["X"], w["Y"], angles, w["Z"], **kw)
+
+
+def _defocus_steps(df, ndf, width, centre, ps):
+    # correct.py:44-93 (automatic step)
+    if ndf == 1:
+        return df, df, 0, 1
+    z0 = df - centre * ps
+    z1 = df + centre * ps
+    z2 = df - (width - centre) * ps
+    z3 = df + (width - centre) * ps
+    mn = min([z0.min(), z1.min(), z2.min(), z3.min()])
+    mx = max([z0.max(), z1.max(), z2.max(), z3.max()])
+    return mn, mx, (mx - mn) / (ndf - 1), ndf
+
+
+def time_path(C, plan, d_in, d_vol, stream, steps, warmup, graph):
+    ev = [C.Event() for _ in range(steps + 1)]
+    g = None
+    if graph:
+        C.call("gx_graph_capture_begin", stream.handle)
+        plan.run(d_in, d_vol, stream)
+        p = C.ctypes.c_void_p()
+        C.call("gx_graph_capture_end", stream.handle, C.ctypes.byref(p))
+        g = p.value
+
+    def step():
+        if g is not None:
+            C.call("gx_graph_launch", g, stream.handle)
+        else:
+            plan.run(d_in, d_vol, stream)
+
+    for _ in range(warmup):
+        step()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(steps):
+        step()
+        ev[i + 1].record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    lat = [ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)]
+    total = ev[0].elapsed_ms(ev[steps])
+    if g is not None:
+        C.call("gx_graph_destroy", g)
+    for e in ev:
+        e.destroy()
+    return total / steps, lat, wall
+
+
+def time_kernel(C, fn, stream, reps):
+    a, b = C.Event(), C.Event()
+    fn()
+    stream.synchronize()
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    stream.synchronize()
+    ms = a.elapsed_ms(b) / reps
+    a.destroy()
+    b.destroy()
+    return ms
+
+
+# ---------------------------------------------------- CPU oracle legs -----
+
+def _oracle_correct_rows(args):
+    """FP32 and FP64 oracle correction of one tilt; only sampled rows kept."""
+    from oracle import oracle as O
+    img, w, rows = args
+    l = O.electron_wavelength(300000)
+    ctf = O.ctf_simple_grid(l, w["df"], 2.7e7, 0, 0, w["phi"], w["X"], w["Y"],
+                            w["ps"]).astype(numpy.complex64)
+    c32 = O.corr(img, ctf, precision="f32")
+    c64 = O.corr(img, ctf, precision="f64")
+    return c32[rows], c64[rows]
+
+
+def oracle_check(w, proj, angles, rows, vols, pool, dump=None):
+    """rel-L2 of GPU volumes at the sampled rows against the FP32 and FP64
+    oracle chains (correct_projections + reconstruct)."""
+    from oracle import oracle as O
+    T = proj.shape[0]
+    if w["df"] is None:
+        s32 = numpy.ascontiguousarray(proj[:, rows, :])
+        s64 = s32.astype(numpy.float64)
+    else:
+        res = pool.map(_oracle_correct_rows,
+                       [(proj[t], w, rows) for t in range(T)])
+        s32 = numpy.stack([r[0] for r in res])  # (T, nrows, X)
+        s64 = numpy.stack([r[1] for r in res])
+    out = {}
+    ref32 = numpy.stack([O.recon(numpy.ascontiguousarray(s32[:, i, :]),
+                                 (w["Z"], w["X"]), angles, w["X"] / 2.0,
+                                 w["ps"]) for i in range(len(rows))])
+    ref64 = numpy.stack([O.recon(numpy.ascontiguousarray(s64[:, i, :]),
+                                 (w["Z"], w["X"]), angles, w["X"] / 2.0,
+                                 w["ps"], precision="f64")
+                         for i in range(len(rows))])
+    if dump:
+        os.makedirs(dump, exist_ok=True)
+        numpy.save(os.path.join(dump, "s32.npy"), s32)
+        numpy.save(os.path.join(dump, "ref32.npy"), ref32)
+        for name, v in vols.items():
+            numpy.save(os.path.join(dump, "vol_%s.npy" % name), v)
+        numpy.save(os.path.join(dump, "proj_t0.npy"), proj[0])
+    for name, v in vols.items():
+        v = v.astype(numpy.float64)
+        out[name + "_vs_fp64"] = float(numpy.linalg.norm(v - ref64) /
+                                       numpy.linalg.norm(ref64))
+        if name == "f32":
+            out["f32_vs_fp32_oracle"] = float(
+                numpy.linalg.norm(v - ref32) / numpy.linalg.norm(ref32))
+    return out
+
+
+def cpu_baseline(w, proj, angles, budget_s):
+    """Oracle (one core) on a bounded sample: some tilt corrections and some
+    y-slices of filter + backprojection; extrapolated to the whole job."""
+    from oracle import oracle as O
+    T, Y, X, Z = w["T"], w["Y"], w["X"], w["Z"]
+    l = O.electron_wavelength(300000)
+    t_corr = 0.0
+    n_corr = 0
+    if w["df"] is not None:
+        ctf = O.ctf_simple_grid(l, w["df"], 2.7e7, 0, 0, w["phi"], X, Y,
+                                w["ps"]).astype(numpy.complex64)
+        t0 = time.perf_counter()
+        while n_corr < T and (n_corr < 2 or
+                              time.perf_counter() - t0 < 0.3 * budget_s):
+            O.corr(proj[n_corr], ctf, precision="f32")
+            n_corr += 1
+        t_corr = (time.perf_counter() - t0) / n_corr
+    n_sl = 0
+    t0 = time.perf_counter()
+    ys = numpy.linspace(0, Y - 1, 64).astype(int)
+    while n_sl < len(ys) and (n_sl < 2 or
+                              time.perf_counter() - t0 < 0.7 * budget_s):
+        sino = numpy.ascontiguousarray(proj[:, ys[n_sl], :])
+        O.recon(sino, (Z, X), angles, X / 2.0, w["ps"])
+        n_sl += 1
+    t_slice = (time.perf_counter() - t0) / n_sl
+    total = t_corr * T + t_slice * Y
+    return {
+        "value": X * Y * Z * T / total,
+        "unit": UNIT,
+        "cores": 1,
+        "kind": "port",
+        "sample": "%d of %d tilt corrections (%.3f s each) and %d of %d "
+                  "y-slices of filter+backprojection (%.3f s each), "
+                  "extrapolated linearly; project-written FFT in place of "
+                  "FFTW" % (n_corr, T, t_corr, n_sl, Y, t_slice),
+        "seconds_estimated_whole_job": total,
+    }
+
+
+def load_traffic(name):
+    """HBM bytes per BP launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+# ------------------------------------------------------------------ main --
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--storage", default="f16", choices=["f16", "bf16"])
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--dump", default=None,
+                    help="directory to save the arrays the oracle check "
+                         "compares")
+    ap.add_argument("--only", choices=["plain", "batched"], default=None,
+                    help="time one path only")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="no timing: run the filter pass once, the "
+                         "backprojection twice and a 256 MiB copy kernel "
+                         "twice (for rocprofv3 traces and counters)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    w = WORKLOADS[args.config]
+    check = not args.no_check and args.only is None and w["ndf"] == 1
+    # CPU workers for the oracle check are forked before anything touches
+    # the GPU, so they inherit no device state
+    pool = None
+    if check:
+        pool = mp.get_context("fork").Pool(min(16, os.cpu_count() or 1))
+
+    from guanaco_amd import _capi as C
+    if C.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    X, Y, T, Z, ndf = w["X"], w["Y"], w["T"], w["Z"], w["ndf"]
+    angles = numpy.deg2rad(
+        numpy.arange(T) * w["step"] - 0.5 * (T - 1) * w["step"]).astype(
+        numpy.float32)
+    updates = float(X) * Y * Z * T
+    stream = C.Stream()
+    print("bench: %s on %s" % (args.config, C.device_info(0)["name"]),
+          file=sys.stderr, flush=True)
+
+    # synthetic input, generated and kept in HBM
+    from oracle import oracle as O
+    d_in = C.DeviceArray((T, Y, X))
+    ctf = None
+    if w["df"] is not None:
+        ctf = dict(wavelength=O.electron_wavelength(300000), Cs=2.7e7,
+                   defocus=w["df"], phi=w["phi"], pixel_size=w["ps"])
+    C.synth_projections(d_in, X, Y, angles, depth=Z, width=X,
+                        num_ellipsoids=w["ne"], seed=w["seed"], semi=w["semi"],
+                        noise="poisson" if w["dose"] else "gaussian",
+                        noise_level=w["dose"] if w["dose"] else 0.01,
+                        ctf=ctf, stream=stream)
+    d_vol = C.DeviceArray((Y, Z, X))
+    d_vol2 = C.DeviceArray((Y, Z, X))
+    if args.profile_kernels:
+        for storage in (("f32", args.storage) if args.only is None else
+                        (("f32",) if args.only == "plain" else
+                         (args.storage,))):
+            plan = make_plans(C, w, angles, storage)
+            plan.filter(d_in, stream)
+            plan.backproject(d_vol, stream)
+            plan.backproject(d_vol, stream)
+            stream.synchronize()
+            plan.destroy()
+        n = 64 << 20
+        a, b = C.DeviceArray((n,), numpy.uint32), C.DeviceArray((n,),
+                                                                numpy.uint32)
+        for _ in range(2):
+            C.call("gx_copy_u32", b.ptr, a.ptr, n, stream.handle)
+        stream.synchronize()
+        print("bench: profile-kernels done (copy kernel: %d bytes read, %d "
+              "written per launch)" % (4 * n, 4 * n), file=sys.stderr)
+        return
+
+    results = {}
+    plans = {}
+    for path, storage, graph, vol in (("plain", "f32", False, d_vol),
+                                      ("batched", args.storage, True,
+                                       d_vol2)):
+        if args.only and args.only != path:
+            continue
+        plan = make_plans(C, w, angles, storage)
+        plans[path] = plan
+        ms, lat, wall = time_path(C, plan, d_in, vol, stream, args.steps,
+                                  args.warmup, graph)
+        filt_ms = time_kernel(C, lambda: plan.filter(d_in, stream), stream, 3)
+        bp_ms = time_kernel(C, lambda: plan.backproject(vol, stream), stream,
+                            max(3, args.steps))
+        plan.run(d_in, vol, stream)
+        stream.synchronize()
+        results[path] = dict(storage=storage, ms_per_step=ms,
+                             value=updates / (ms * 1e-3),
+                             p50_ms=percentile(lat, 50),
+                             p99_ms=percentile(lat, 99), wall_s=wall,
+                             filter_ms=filt_ms, bp_ms=bp_ms,
+                             bp_updates_per_s=updates / (bp_ms * 1e-3))
+        print("bench: %s (%s) %.2f ms/step, BP %.2f ms, filter %.2f ms"
+              % (path, storage, ms, bp_ms, filt_ms), file=sys.stderr,
+              flush=True)
+
+    head = results.get("batched") or results["plain"]
+    s_bytes = 2 if head["storage"] in ("f16", "bf16") else 4
+    b_alg = 2 * s_bytes + 4.0 / T
+    achieved = updates * b_alg / (head["bp_ms"] * 1e-3) / 1e9
+    traffic = load_traffic("bp_%s_%s_pmc.txt" % (args.config, head["storage"]))
+    line = {
+        "metric": METRIC,
+        "value": head["value"],
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded %d-ellipsoid phantom, analytic "
+                "projections%s, %s (seed %d)" % (
+                    w["ne"], ", CTF forward model" if w["df"] else "",
+                    "Poisson %g e-/px" % w["dose"] if w["dose"]
+                    else "Gaussian noise 1%% of max", w["seed"]),
+        "config": {
+            "workload": w["name"],
+            "path": "batched" if "batched" in results else "plain",
+            "storage": head["storage"],
+            "accumulate": "f32",
+            "volume": [Y, Z, X],
+            "tilts": T,
+            "num_defocus": ndf,
+            "updates_per_step": updates,
+        },
+        "latency_ms": {"p50": head["p50_ms"], "p99": head["p99_ms"]},
+        "paths": results,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_bp (backprojection, %s slabs)" % head["storage"],
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_update": b_alg,
+            "updates_per_launch": updates,
+            "launch_ms": head["bp_ms"],
+        },
+    }
+    if "batched" in results and "plain" in results:
+        line["speedup_batched_vs_plain"] = (results["batched"]["value"] /
+                                            results["plain"]["value"])
+    if check:
+        rows = numpy.linspace(0, Y - 1, 4).astype(int)
+        proj = d_in.to_host()
+        v32 = d_vol.to_host()[rows]
+        v16 = d_vol2.to_host()[rows]
+        t0 = time.perf_counter()
+        line["rel_l2"] = oracle_check(
+            w, proj, angles, rows, {"f32": v32, args.storage: v16}, pool,
+            dump=args.dump)
+        pool.close()
+        line["rel_l2"]["sample"] = "y-slices %s" % rows.tolist()
+        print("bench: oracle check %.1f s %s" % (time.perf_counter() - t0,
+                                                line["rel_l2"]),
+              file=sys.stderr, flush=True)
+    if not args.no_cpu:
+        proj = d_in.to_host()
+        line["cpu_baseline"] = cpu_baseline(w, proj, angles, args.cpu_budget)
+    for p in plans.values():
+        p.destroy()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

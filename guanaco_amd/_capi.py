@@ -68,6 +68,22 @@ class VolumeDesc(ctypes.Structure):
     ]
 
 
+class SynthDesc(ctypes.Structure):
+    _fields_ = [
+        ("num_pixels", _sz), ("num_rows", _sz), ("num_angles", _sz),
+        ("depth", _sz), ("width", _sz), ("centre", ctypes.c_float),
+        ("angles", _f32p), ("num_ellipsoids", _sz),
+        ("seed", ctypes.c_uint64),
+        ("semi_min", ctypes.c_float), ("semi_max", ctypes.c_float),
+        ("density_min", ctypes.c_float), ("density_max", ctypes.c_float),
+        ("noise", _int), ("noise_level", ctypes.c_float),
+        ("contrast", ctypes.c_float), ("ctf", _int),
+        ("wavelength", ctypes.c_double), ("Cs", ctypes.c_double),
+        ("defocus", ctypes.c_double), ("phi", ctypes.c_double),
+        ("pixel_size", ctypes.c_double),
+    ]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
     "gx_last_error": [],
@@ -84,6 +100,7 @@ SIGNATURES = {
     "gx_memcpy_dtoh": [_vp, _vp, _sz, _vp],
     "gx_memcpy_dtod": [_vp, _vp, _sz, _vp],
     "gx_fill_u32": [_vp, ctypes.c_uint32, _sz, _vp],
+    "gx_copy_u32": [_vp, _vp, _sz, _vp],
     "gx_stream_create": [ctypes.POINTER(_vp)],
     "gx_stream_destroy": [_vp],
     "gx_stream_synchronize": [_vp],
@@ -110,6 +127,7 @@ SIGNATURES = {
     "gx_volume_backproject": [_vp, _vp, _vp],
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
+    "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],
 }
 _RESTYPE = {"gx_last_error": ctypes.c_char_p}
 
@@ -355,3 +373,28 @@ def plan_bytes(**kw):
     n = _sz()
     call("gx_volume_plan_bytes", ctypes.byref(d), ctypes.byref(n))
     return n.value
+
+
+def synth_projections(d_out, num_pixels, num_rows, angles, depth=0, width=0,
+                      centre=None, num_ellipsoids=20, seed=1000,
+                      semi=(5.0, 40.0), density=(0.1, 1.0), noise=None,
+                      noise_level=0.0, contrast=0.2, ctf=None, stream=None):
+    """Fill d_out (T, Y, X) with a seeded synthetic tilt series.
+
+    noise: None, "gaussian" (sigma = noise_level * max) or "poisson" (mean
+    noise_level electrons per pixel). ctf: None or a dict with wavelength,
+    Cs, defocus (A), phi (rad) and pixel_size (A).
+    """
+    angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
+    mode = {None: 0, "gaussian": 1, "poisson": 2}[noise]
+    c = ctf or {}
+    d = SynthDesc(num_pixels, num_rows, angles.size, depth, width,
+                  float(num_pixels / 2.0 if centre is None else centre),
+                  f32p(angles), num_ellipsoids, seed, float(semi[0]),
+                  float(semi[1]), float(density[0]), float(density[1]), mode,
+                  float(noise_level), float(contrast), 1 if ctf else 0,
+                  float(c.get("wavelength", 0.0)), float(c.get("Cs", 0.0)),
+                  float(c.get("defocus", 0.0)), float(c.get("phi", 0.0)),
+                  float(c.get("pixel_size", 1.0)))
+    call("gx_synth_projections", ctypes.byref(d), d_out.ptr,
+         stream.handle if isinstance(stream, Stream) else stream)

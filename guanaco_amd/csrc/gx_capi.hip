@@ -274,6 +274,13 @@ extern "C" int gx_fill_u32(void *dst, uint32_t value, size_t count,
   return GX_OK;
 }
 
+extern "C" int gx_copy_u32(void *dst, const void *src, size_t count,
+                           void *stream) {
+  GX_HIP(launch_copy_u32((uint32_t *)dst, (const uint32_t *)src, count,
+                         (hipStream_t)stream));
+  return GX_OK;
+}
+
 extern "C" int gx_stream_create(void **stream) {
   GX_CHECK(stream != nullptr);
   hipStream_t s;
@@ -799,5 +806,48 @@ extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
   if (d_slab) *d_slab = p->slab.p;
   if (bytes) *bytes = p->slab_bytes;
   if (rows_per_block) *rows_per_block = p->YB;
+  return GX_OK;
+}
+
+// ------------------------------------------------------- synthetic data ---
+
+extern "C" int gx_synth_projections(const gx_synth_desc *d, float *d_out,
+                                    void *stream) {
+  GX_CHECK(d != nullptr && d_out != nullptr && d->angles != nullptr);
+  GX_CHECK(d->num_pixels > 0 && d->num_rows > 0 && d->num_angles > 0);
+  GX_CHECK(d->num_ellipsoids <= 512);
+  GX_CHECK(d->noise >= 0 && d->noise <= 2);
+  if (d->ctf) {
+    int rc = check_corr_sizes(d->num_pixels, d->num_rows);
+    if (rc) return rc;
+    GX_CHECK(d->pixel_size > 0);
+  }
+  SynthArgs a{};
+  a.num_pixels = d->num_pixels;
+  a.num_rows = d->num_rows;
+  a.num_angles = d->num_angles;
+  a.depth = d->depth;
+  a.width = d->width;
+  a.centre = d->centre;
+  a.angles = d->angles;
+  a.num_ellipsoids = d->num_ellipsoids;
+  a.seed = d->seed;
+  a.semi_min = d->semi_min;
+  a.semi_max = d->semi_max;
+  a.density_min = d->density_min;
+  a.density_max = d->density_max;
+  a.noise = d->noise;
+  a.noise_level = d->noise_level;
+  a.contrast = d->contrast;
+  a.ctf = d->ctf;
+  a.wavelength = d->wavelength;
+  a.Cs = d->Cs;
+  a.defocus = d->defocus;
+  a.phi = d->phi;
+  a.pixel_size = d->pixel_size;
+  hipError_t terr;
+  twiddles(&terr);
+  GX_HIP(terr);
+  GX_HIP(synth_projections(&a, d_out, (hipStream_t)stream));
   return GX_OK;
 }

@@ -9,6 +9,17 @@
 
 namespace gx {
 
+// Workgroup barrier that orders every LDS access before it against every
+// access after it. The empty asm statements with a memory clobber stop the
+// compiler from moving loads or stores across the barrier: hipcc has sunk
+// LDS loads below __syncthreads() (into a following loop) so that a fast wave
+// overwrote the data a slower wave had still to read.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  asm volatile("" ::: "memory");
+}
+
 // Spectrum columns per block in the column-blocked half-spectrum layout
 // F[t][cb][y][ci] (kx = cb * CB + ci). A block of CB columns of four
 // consecutive rows is one 128-byte segment.
@@ -85,12 +96,12 @@ __device__ __forceinline__ float block_max(float v, float *scratch) {
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) scratch[wave] = v;
-  __syncthreads();
+  lds_barrier();
   float r = 0.0f;
   constexpr int NW = (NTHREADS + 63) / 64;
 #pragma unroll
   for (int w = 0; w < NW; ++w) r = fmaxf(r, scratch[w]);
-  __syncthreads();
+  lds_barrier();
   return r;
 }
 

@@ -33,6 +33,24 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Plain copy, the calibration kernel for HBM byte counters.
+__global__ void k_copy_u32(uint32_t *__restrict__ dst,
+                           const uint32_t *__restrict__ src, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, size_t n,
+                           hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)blocks), dim3(256), 0, s, dst,
+                     src, n);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------- CTF masks --
 
 // The reference stores H = exp(-i(chi - phi)) as complex64 (correct.py:192)
@@ -162,7 +180,7 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
   Fft<X, E, -1, 1>::run(v, s[p], tid, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) s[p][tid + m * NT] = v[m];
-  __syncthreads();
+  lds_barrier();
   if (tmax != nullptr) {
     amax = block_max<2 * NT>(amax, red);
     if (threadIdx.x == 0) atomicMax(&tmax[t0 + t], __float_as_uint(amax));
@@ -187,10 +205,20 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
 
 // One block of CB columns (CB * Y / E threads). The forward column transform
 // stays in registers; every defocus k reuses it: multiply by the mask,
-// inverse transform, store G[t][k][cb][y][ci].
-template <int Y>
+// inverse transform, store G[t][k][cb][y][ci]. M = int8_t: the flip mask
+// (M(k) + M(-k), scaled by 1/2); M = float: a real transfer function.
+template <typename M>
+__device__ __forceinline__ float mask_value(M m) {
+  if constexpr (sizeof(M) == 1) {
+    return 0.5f * (float)m;
+  } else {
+    return (float)m;
+  }
+}
+
+template <int Y, typename M>
 __global__ __launch_bounds__(CB *(Y / default_e(Y))) void k_cols(
-    const float2 *__restrict__ F, const int8_t *__restrict__ mask,
+    const float2 *__restrict__ F, const M *__restrict__ mask,
     float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw) {
   constexpr int E = default_e(Y);
   constexpr int NT = Y / E;
@@ -203,11 +231,11 @@ __global__ __launch_bounds__(CB *(Y / default_e(Y))) void k_cols(
   for (int m = 0; m < E; ++m) v[m] = src[(size_t)(tid + m * NT) * CB + ci];
   Fft<Y, E, -1, CB>::run(v, s + ci, tid, tw);
   for (int k = 0; k < ndf; ++k) {
-    const int8_t *mk = mask + ((size_t)k * XB + cb) * Y * CB;
+    const M *mk = mask + ((size_t)k * XB + cb) * Y * CB;
     float2 w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const float f = 0.5f * (float)mk[(size_t)(tid + m * NT) * CB + ci];
+      const float f = mask_value(mk[(size_t)(tid + m * NT) * CB + ci]);
       w[m] = make_float2(v[m].x * f, v[m].y * f);
     }
     Fft<Y, E, 1, CB>::run(w, s + ci, tid, tw);
@@ -269,12 +297,12 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   Fft<X, EX, 1, 1>::run(v, s[p], tid, tw);  // v = a + i b (unnormalised)
 #pragma unroll
   for (int m = 0; m < EX; ++m) s[p][tid + m * NT] = v[m];
-  __syncthreads();
+  lds_barrier();
   float2 u[EL];
 #pragma unroll
   for (int m = 0; m < EL; ++m)
     u[m] = m < EL / 2 ? s[p][tid + m * NT] : make_float2(0.0f, 0.0f);
-  __syncthreads();
+  lds_barrier();
   Fft<L, EL, -1, 1>::run(u, s[p], tid, tw);
 #pragma unroll
   for (int m = 0; m < EL; ++m) {
@@ -419,17 +447,19 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
   return hipGetLastError();
 }
 
-hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
-                       int XB, int ndf, int ntilts, hipStream_t s) {
+template <typename M>
+static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
+                                int Y, int XB, int ndf, int ntilts,
+                                hipStream_t s) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
   dim3 grid(XB, ntilts);
   switch (Y) {
-#define C_(n)                                                                 \
-  case n:                                                                     \
-    hipLaunchKernelGGL(k_cols<n>, grid, dim3(CB * (n / default_e(n))), 0, s,  \
-                       F, mask, G, XB, ndf, tw);                              \
+#define C_(n)                                                              \
+  case n:                                                                  \
+    hipLaunchKernelGGL((k_cols<n, M>), grid, dim3(CB * (n / default_e(n))), \
+                       0, s, F, mask, G, XB, ndf, tw);                     \
     break;
     GX_POW2_CASES(C_)
 #undef C_
@@ -437,6 +467,17 @@ hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
+                       int XB, int ndf, int ntilts, hipStream_t s) {
+  return launch_cols_t<int8_t>(F, mask, G, Y, XB, ndf, ntilts, s);
+}
+
+hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
+                                float2 *G, int Y, int XB, int ntilts,
+                                hipStream_t s) {
+  return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s);
 }
 
 template <int STORE>

@@ -9,7 +9,7 @@
 // exchange through LDS. Several transforms can share a workgroup; each uses
 // its own LDS region, addressed as lds[i * LS] so that columns can be
 // interleaved. Every thread of the workgroup must call run() (it contains
-// __syncthreads()).
+// workgroup barriers).
 //
 // Conventions follow FFTW (the reference's host FFT, src/libguanaco/
 // fft.cpp:38-71): DIR = -1 is exp(-2 pi i jk/N), DIR = +1 is exp(+...), both
@@ -18,6 +18,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include "gx_common.hpp"
 
 namespace gx {
 
@@ -209,7 +211,7 @@ struct Fft {
 #pragma unroll
         for (int m = 0; m < R; ++m) w[q * R + m] = lds[(b + m * NB) * LS];
       }
-      __syncthreads();
+      lds_barrier();
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int b = tid + q * NT;
@@ -222,7 +224,7 @@ struct Fft {
 #pragma unroll
         for (int m = 0; m < R; ++m) lds[(base + m * Ns) * LS] = w[q * R + m];
       }
-      __syncthreads();
+      lds_barrier();
       middle<S + 1>(lds, tid, tw);
     }
   }
@@ -237,11 +239,11 @@ struct Fft {
       dft<E, DIR>(v);
 #pragma unroll
       for (int m = 0; m < E; ++m) lds[(tid * E + m) * LS] = v[m];
-      __syncthreads();
+      lds_barrier();
       middle<1>(lds, tid, tw);
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = lds[(tid + m * NT) * LS];
-      __syncthreads();
+      lds_barrier();
       constexpr int TWS = kTwN / N;
 #pragma unroll
       for (int m = 1; m < E; ++m)

@@ -20,6 +20,8 @@ struct BpArgs {
 
 // Each returns hipSuccess or the launch error.
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, size_t n,
+                           hipStream_t s);
 
 hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
                            const double *d_defocus, double l, double Cs,
@@ -33,6 +35,10 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
                            int t0, hipStream_t s);
 hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
                        int XB, int ndf, int ntilts, hipStream_t s);
+// Same column pass with a real transfer function (one "defocus").
+hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
+                                float2 *G, int Y, int XB, int ntilts,
+                                hipStream_t s);
 hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   int ndf, int t0, int ntilts, int T,
                                   const float *ramp, const float *wscale,
@@ -52,6 +58,21 @@ hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
 
 // store: kF32 (exact reference arithmetic) or kF16/kBF16 (FP32 accumulate).
 hipError_t launch_bp(const BpArgs &a, int YB, int store, hipStream_t s);
+
+// Seeded synthetic tilt series (ellipsoid phantom, optional CTF, noise).
+struct SynthArgs {
+  size_t num_pixels, num_rows, num_angles, depth, width;
+  float centre;
+  const float *angles;  // host
+  size_t num_ellipsoids;
+  uint64_t seed;
+  float semi_min, semi_max, density_min, density_max;
+  int noise;  // 0 none, 1 Gaussian, 2 Poisson
+  float noise_level, contrast;
+  int ctf;
+  double wavelength, Cs, defocus, phi, pixel_size;
+};
+hipError_t synth_projections(const SynthArgs *a, float *d_out, hipStream_t s);
 
 // Twiddle table W[j] = exp(-2 pi i j / kTwN) for the current device.
 const float2 *twiddles(hipError_t *err);

@@ -57,6 +57,8 @@ int gx_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
 int gx_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
 /* Fill with a 32-bit pattern by kernel (safe under stream capture). */
 int gx_fill_u32(void *dst, uint32_t value, size_t count, void *stream);
+/* Plain element-wise copy kernel (also the byte-counter calibration). */
+int gx_copy_u32(void *dst, const void *src, size_t count, void *stream);
 int gx_stream_create(void **stream);
 int gx_stream_destroy(void *stream);
 int gx_stream_synchronize(void *stream);
@@ -172,6 +174,41 @@ int gx_volume_backproject(gx_volume_plan *plan, float *d_vol, void *stream);
 /* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage. */
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);
+
+/* ---- seeded synthetic workloads (no reference counterpart) --------------- */
+
+/* Tilt series of a random-ellipsoid phantom projected analytically in the
+ * reconstruction geometry, optionally multiplied by the CTF sin(chi - phi)
+ * in Fourier space, with Gaussian or Poisson noise; every value depends on
+ * `seed` only. Writes (num_angles x num_rows x num_pixels) float32 to d_out
+ * (device). Synchronous. */
+typedef struct gx_synth_desc {
+  size_t num_pixels;     /* X */
+  size_t num_rows;       /* Y */
+  size_t num_angles;     /* T */
+  size_t depth;          /* Z of the phantom box (0 = X) */
+  size_t width;          /* W of the phantom box (0 = X) */
+  float centre;          /* rotation centre, pixels */
+  const float *angles;   /* host, T radians */
+  size_t num_ellipsoids; /* <= 512 */
+  uint64_t seed;
+  float semi_min, semi_max;       /* semi-axes, pixels */
+  float density_min, density_max; /* densities */
+  int noise;         /* 0 none; 1 Gaussian, sigma = noise_level * max;
+                        2 Poisson, mean = noise_level * (1 + contrast *
+                        signal / max|signal|) */
+  float noise_level;
+  float contrast;
+  int ctf;           /* apply the CTF (X, Y powers of two) */
+  double wavelength; /* A */
+  double Cs;         /* A */
+  double defocus;    /* A */
+  double phi;        /* rad */
+  double pixel_size; /* A */
+} gx_synth_desc;
+
+int gx_synth_projections(const gx_synth_desc *desc, float *d_out,
+                         void *stream);
 
 #ifdef __cplusplus
 }

@@ -212,3 +212,92 @@ def test_volume_sinogram_layout_and_weights(capi):
                            sinogram_order=True, transform="minus",
                            angular_weights=True, depth=Z)
     assert rel_l2(got, expect) < 1e-5
+
+
+# ------------------------------------------- graphs and synthetic data ----
+
+def test_graph_replays_match_eager(capi):
+    # The whole request captured once and replayed several times, with eager
+    # work and synchronisations in between; every replay must equal the
+    # eager result bit for bit.
+    X, Y, T, ndf, Z, ps, df = 64, 16, 9, 3, 16, 1.5, 30000.0
+    proj, angles, centre, dfs, rel = ctf_problem(X, Y, T, ndf, ps, df, 5)
+    l = O.electron_wavelength(300000)
+    for storage in ("f32", "f16"):
+        plan = C.VolumePlan(X, Y, angles, Z, num_defocus=ndf, pixel_size=ps,
+                            min_defocus=rel[0], max_defocus=rel[1],
+                            storage=storage, defocus=dfs, wavelength=l,
+                            Cs=2.7e7)
+        s = C.Stream()
+        d_in = C.DeviceArray.from_host(proj)
+        d_eager = C.DeviceArray((Y, Z, X))
+        d_graph = C.DeviceArray((Y, Z, X))
+        plan.run(d_in, d_eager, s)
+        s.synchronize()
+        eager = d_eager.to_host()
+        C.call("gx_graph_capture_begin", s.handle)
+        plan.run(d_in, d_graph, s)
+        g = C.ctypes.c_void_p()
+        C.call("gx_graph_capture_end", s.handle, C.ctypes.byref(g))
+        for rep in range(4):
+            C.call("gx_fill_u32", d_graph.ptr, 0x7fc00000, Y * Z * X, s.handle)
+            C.call("gx_graph_launch", g.value, s.handle)
+            s.synchronize()
+            numpy.testing.assert_array_equal(d_graph.to_host(), eager)
+            plan.run(d_in, d_eager, s)  # eager work between replays
+            s.synchronize()
+        C.call("gx_graph_destroy", g.value)
+        s.destroy()
+        plan.destroy()
+
+
+def test_synthetic_projections_are_seeded(capi):
+    X, Y, T = 64, 32, 5
+    angles = tilts(T)
+    a = C.DeviceArray((T, Y, X))
+    b = C.DeviceArray((T, Y, X))
+    ctf = dict(wavelength=O.electron_wavelength(300000), Cs=2.7e7,
+               defocus=20000.0, phi=math.asin(0.1), pixel_size=2.0)
+    C.synth_projections(a, X, Y, angles, depth=16, num_ellipsoids=10,
+                        seed=7, noise="poisson", noise_level=50.0, ctf=ctf)
+    C.synth_projections(b, X, Y, angles, depth=16, num_ellipsoids=10,
+                        seed=7, noise="poisson", noise_level=50.0, ctf=ctf)
+    ha, hb = a.to_host(), b.to_host()
+    numpy.testing.assert_array_equal(ha, hb)
+    assert numpy.all(ha >= 0) and numpy.all(ha == numpy.round(ha))
+    assert 40 < ha.mean() < 60
+    # no ellipsoids, no noise: an empty tilt series
+    C.synth_projections(a, X, Y, angles, depth=X, num_ellipsoids=0, seed=1)
+    assert numpy.all(a.to_host() == 0)
+
+
+@pytest.mark.parametrize("chunk", [16, 0])
+def test_identical_tilts_give_identical_slabs(capi, chunk):
+    # Regression: with many workgroups resident per CU, LDS loads that the
+    # compiler had moved below a workgroup barrier read data another wave
+    # had already overwritten, corrupting one column block of a few tilts
+    # per run. Identical images in every tilt must give bit-identical slabs,
+    # run after run.
+    X = Y = 1024
+    T, Z = 48, 8
+    img = numpy.random.default_rng(0).poisson(50, (Y, X)).astype(
+        numpy.float32)
+    d_in = C.DeviceArray.from_host(numpy.broadcast_to(img, (T, Y, X)).copy())
+    l = O.electron_wavelength(300000)
+    for storage in ("f32", "f16"):
+        plan = C.VolumePlan(X, Y, numpy.zeros(T, numpy.float32), Z,
+                            pixel_size=2.0, storage=storage,
+                            defocus=numpy.array([20000.0]), wavelength=l,
+                            Cs=2.7e7, phi=math.asin(0.1), tilt_chunk=chunk)
+        s = C.Stream()
+        for run in range(4):
+            plan.filter(d_in, s)
+            s.synchronize()
+            p, n, yb = plan.slabs()
+            a = numpy.empty(n // 4, numpy.uint32)
+            C.call("gx_memcpy_dtoh", a.ctypes.data, p, n, None)
+            a = a.reshape(T, -1)
+            bad = [t for t in range(T) if not numpy.array_equal(a[t], a[0])]
+            assert not bad, (storage, run, bad)
+        s.destroy()
+        plan.destroy()
