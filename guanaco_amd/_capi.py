@@ -125,6 +125,8 @@ SIGNATURES = {
     "gx_volume_run": [_vp, _vp, _vp, _vp],
     "gx_volume_filter": [_vp, _vp, _vp],
     "gx_volume_backproject": [_vp, _vp, _vp],
+    "gx_volume_backproject_rows": [_vp, _sz, _sz, _vp, _vp],
+    "gx_volume_rows_per_block": [_vp, ctypes.POINTER(_sz)],
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
     "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],

@@ -93,12 +93,12 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
     const typename StoreT<STORE>::T *__restrict__ slab,
     const float4 *__restrict__ coef, const uint32_t *__restrict__ tmax,
     float nrm_mul, float dscale, float doffset, int ndf, int N, int T,
-    int NYB, int W, int Z, int Y, float *__restrict__ vol, int nbx, int nbz,
-    unsigned nblocks) {
+    int NYB, int W, int Z, int Y, int yb0, int y_base,
+    float *__restrict__ vol, int nbx, int nbz, unsigned nblocks) {
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
   const int bz = (int)(lb % (unsigned)nbz);
   const int bx = (int)((lb / (unsigned)nbz) % (unsigned)nbx);
-  const int byb = (int)(lb / ((unsigned)nbz * (unsigned)nbx));
+  const int byb = yb0 + (int)(lb / ((unsigned)nbz * (unsigned)nbx));
   const int ix = bx * kTX + (int)(threadIdx.x % kTX);
   const int iz = bz * kTZ + (int)(threadIdx.x / kTX);
   const bool active = ix < W && iz < Z;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
 #pragma unroll
     for (int r = 0; r < YB; ++r) {
       const int y = byb * YB + r;
-      if (y < Y) vol[((size_t)y * Z + iz) * W + ix] = acc[r];
+      if (y < Y) vol[((size_t)(y - y_base) * Z + iz) * W + ix] = acc[r];
     }
   }
 }
@@ -170,17 +170,19 @@ static hipError_t launch_bp_t(const BpArgs &a, hipStream_t s) {
   using ST = typename StoreT<STORE>::T;
   const int nbx = (a.W + kTX - 1) / kTX;
   const int nbz = (a.Z + kTZ - 1) / kTZ;
-  const unsigned nb = (unsigned)nbx * (unsigned)nbz * (unsigned)a.NYB;
+  if (a.yb1 <= a.yb0) return hipSuccess;
+  const unsigned nb =
+      (unsigned)nbx * (unsigned)nbz * (unsigned)(a.yb1 - a.yb0);
   if (a.ndf > 1) {
     hipLaunchKernelGGL((k_bp<YB, STORE, true>), dim3(nb), dim3(kTX * kTZ), 0,
                        s, (const ST *)a.slab, a.coef, a.tmax, a.nrm_mul,
                        a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
-                       a.Y, a.vol, nbx, nbz, nb);
+                       a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
   } else {
     hipLaunchKernelGGL((k_bp<YB, STORE, false>), dim3(nb), dim3(kTX * kTZ), 0,
                        s, (const ST *)a.slab, a.coef, a.tmax, a.nrm_mul,
                        a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
-                       a.Y, a.vol, nbx, nbz, nb);
+                       a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
   }
   return hipGetLastError();
 }

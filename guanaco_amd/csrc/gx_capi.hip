@@ -414,6 +414,9 @@ static int bp_slice(const gx_recon_cfg *c, const float *d_slab, float *d_rec,
   a.W = (int)c->grid_width;
   a.Z = (int)c->grid_height;
   a.Y = 1;
+  a.yb0 = 0;
+  a.yb1 = 1;
+  a.y_base = 0;
   a.vol = d_rec;
   GX_HIP(launch_bp(a, 1, kF32, s));
   GX_HIP(hipStreamSynchronize(s));
@@ -771,9 +774,13 @@ extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
   return GX_OK;
 }
 
-extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
-                                     void *stream) {
+extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
+                                          size_t y1, float *d_vol,
+                                          void *stream) {
   GX_CHECK(p != nullptr && d_vol != nullptr);
+  GX_CHECK(y0 <= y1 && y1 <= (size_t)p->Y);
+  GX_CHECK(y0 % (size_t)p->YB == 0);
+  GX_CHECK(y1 % (size_t)p->YB == 0 || y1 == (size_t)p->Y);
   BpArgs a{};
   a.slab = p->slab.p;
   a.coef = p->coef.as<float4>();
@@ -787,9 +794,24 @@ extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
   a.NYB = p->NYB;
   a.W = p->W;
   a.Z = p->Z;
-  a.Y = p->Y;
+  a.Y = (int)y1;
+  a.yb0 = (int)(y0 / p->YB);
+  a.yb1 = (int)((y1 + p->YB - 1) / p->YB);
+  a.y_base = (int)y0;
   a.vol = d_vol;
   GX_HIP(launch_bp(a, p->YB, p->storage, (hipStream_t)stream));
+  return GX_OK;
+}
+
+extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
+                                     void *stream) {
+  GX_CHECK(p != nullptr);
+  return gx_volume_backproject_rows(p, 0, (size_t)p->Y, d_vol, stream);
+}
+
+extern "C" int gx_volume_rows_per_block(gx_volume_plan *p, size_t *rows) {
+  GX_CHECK(p != nullptr && rows != nullptr);
+  *rows = (size_t)p->YB;
   return GX_OK;
 }
 

@@ -15,7 +15,9 @@ struct BpArgs {
   float nrm_mul;             // normalisation = float(tmax[t]) * nrm_mul
   float dscale, doffset;     // slab selection (reconstructor.cpp:29-32)
   int ndf, N, T, NYB, W, Z, Y;
-  float *vol;                // [y][z][x]
+  int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
+  int y_base;                // vol holds rows [y_base, ...)
+  float *vol;                // [y - y_base][z][x]
 };
 
 // Each returns hipSuccess or the launch error.

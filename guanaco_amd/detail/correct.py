@@ -1,0 +1,123 @@
+"""CTF correction orchestration (reference src/guanaco/detail/correct.py).
+
+``correct_projections`` keeps the reference's signature, units and return
+value ``(projections, min_defocus - defocus, max_defocus - defocus)``; with
+``device="gpu"`` every tilt is corrected by the gfx950 ``corr`` kernels.
+"""
+import tempfile
+from math import pi
+
+import numpy
+
+import guanaco_amd.detail as ext
+
+__all__ = ["correct_projections", "get_defocus_steps", "get_ctf",
+           "get_ctf_array"]
+
+
+def get_centre(shape, centre, sinogram_order=True):
+    """Per-row rotation centres (correct.py:33-41)."""
+    if not sinogram_order:
+        shape = list(shape)
+        shape[0], shape[-2] = shape[-2], shape[0]
+    if centre is None:
+        centre = numpy.ones(shape[0], dtype="float32") * (shape[-1] / 2.0)
+    elif numpy.array(centre).size == 1:
+        centre = numpy.ones(shape[0], dtype="float32") * centre
+    return numpy.asarray(centre).astype(dtype="float32", copy=False)
+
+
+def get_defocus_steps(defocus, num_defocus=None, step_defocus=None,
+                      width=None, centre=None, pixel_size=None):
+    """Defocus range (min, max, step, num) (correct.py:44-93).
+
+    One defocus: min = max = defocus. With a step: defocus -/+ (n-1)/2 step.
+    Otherwise the range covers the beam heights of the detector edges about
+    the rotation centre, and step = (max - min) / (n - 1).
+    """
+    if num_defocus is None or num_defocus <= 0:
+        num_defocus = 1
+    if num_defocus == 1:
+        return defocus, defocus, 0, 1
+    if step_defocus is not None and step_defocus > 0:
+        half = 0.5 * (num_defocus - 1) * step_defocus
+        return defocus - half, defocus + half, step_defocus, num_defocus
+    centre = numpy.asarray(centre)
+    ends = [defocus - centre * pixel_size, defocus + centre * pixel_size,
+            defocus - (width - centre) * pixel_size,
+            defocus + (width - centre) * pixel_size]
+    mn = min(e.min() for e in ends)
+    mx = max(e.max() for e in ends)
+    return mn, mx, (mx - mn) / (num_defocus - 1), num_defocus
+
+
+def get_corrected_array(shape, filename_or_array=None):
+    """Storage for the corrected stack (correct.py:96-113): a temporary
+    memmap, a named memmap, or the given array reshaped."""
+    if filename_or_array is None:
+        return numpy.memmap(tempfile.TemporaryFile(), mode="w+",
+                            dtype="float32", shape=shape)
+    if isinstance(filename_or_array, str):
+        return numpy.memmap(filename_or_array, mode="w+", dtype="float32",
+                            shape=shape)
+    return filename_or_array.reshape(shape)
+
+
+def get_ctf(width, height, pixel_size, energy, defocus,
+            spherical_aberration=0, astigmatism=0, astigmatism_angle=0,
+            phase_shift=0):
+    """exp(-i(chi - phi)) on the FFT-ordered grid (correct.py:116-154);
+    energy keV, defocus A, Cs mm, Ca A, Pa and phase_shift degrees."""
+    c = ext.CTF(l=ext.get_electron_wavelength(energy * 1000), df=defocus,
+                Cs=spherical_aberration * 1e7, Ca=astigmatism,
+                Pa=astigmatism_angle * pi / 180, phi=phase_shift * pi / 180)
+    return c.get_ctf_simple(width, height, pixel_size)
+
+
+def get_ctf_array(width, height, pixel_size, energy, min_defocus,
+                  num_defocus, step_defocus, spherical_aberration=0,
+                  astigmatism=0, astigmatism_angle=0, phase_shift=0):
+    """complex64 (ndf, height, width) stack for defoci min + d * step
+    (correct.py:157-214)."""
+    out = numpy.zeros((num_defocus, height, width), dtype="complex64")
+    for d in range(num_defocus):
+        out[d] = get_ctf(width, height, pixel_size, energy,
+                         min_defocus + d * step_defocus, spherical_aberration,
+                         astigmatism, astigmatism_angle, phase_shift)
+    return out
+
+
+def correct_projections(projections, centre=None, pixel_size=1, energy=None,
+                        defocus=None, num_defocus=None, step_defocus=None,
+                        spherical_aberration=0, astigmatism=0,
+                        astigmatism_angle=0, phase_shift=0,
+                        corrected_output=None, device="cpu"):
+    """Phase-flip every projection for each defocus (correct.py:217-318).
+
+    Returns (corrected, min_defocus - defocus, max_defocus - defocus); the
+    corrected stack is (T, ndf, Y, X), or (T, Y, X) for one defocus. With
+    defocus None the projections are returned untouched with (0, 0). A
+    missing centre means X/2 for every row (the reference raises TypeError
+    there when num_defocus > 1).
+    """
+    if defocus is None:
+        return projections, 0, 0
+    if centre is None:
+        centre = get_centre(projections.shape, None, sinogram_order=False)
+    mn, mx, step, ndf = get_defocus_steps(defocus, num_defocus, step_defocus,
+                                          projections.shape[2], centre,
+                                          pixel_size)
+    assert abs(mn + (ndf - 1) * step - mx) < 1e-7, "Inconsistent defocus steps"
+    T, Y, X = projections.shape
+    corrected = get_corrected_array((T, ndf, Y, X), corrected_output)
+    ctf = get_ctf_array(X, Y, pixel_size, energy, mn, ndf, step,
+                        spherical_aberration, astigmatism, astigmatism_angle,
+                        phase_shift)
+    buf = numpy.empty((ndf, Y, X), numpy.float32)
+    for z in range(T):
+        image = numpy.ascontiguousarray(projections[z], dtype=numpy.float32)
+        ext.corr(image, ctf, buf, device)
+        corrected[z] = buf
+    if ndf == 1:
+        corrected = corrected.reshape(projections.shape)
+    return corrected, mn - defocus, mx - defocus

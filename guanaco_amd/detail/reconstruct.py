@@ -1,0 +1,159 @@
+"""reconstruct() and its helpers (reference src/guanaco/detail/reconstruct.py).
+
+``reconstruct`` keeps the reference's signature and semantics; with
+``device="gpu"`` the slices go through the gfx950 path, batched into one
+device-resident call. ``reconstruct_projections`` is the fused path: raw
+projections in, CTF correction, filtering and backprojection on the device,
+without materialising the corrected stack on the host.
+"""
+import numpy
+
+import guanaco_amd.detail as ext
+from guanaco_amd.detail import mp
+from guanaco_amd.detail.correct import get_defocus_steps
+
+__all__ = ["reconstruct", "reconstruct_projections", "get_centre",
+           "compute_angular_weights", "get_weights"]
+
+
+def get_centre(shape, centre, sinogram_order=True):
+    """Per-slice rotation centres (reconstruct.py:32-52): X/2 by default, a
+    scalar broadcast to every slice, or the given array."""
+    if not sinogram_order:
+        shape = list(shape)
+        shape[0], shape[-2] = shape[-2], shape[0]
+    if centre is None:
+        centre = numpy.ones(shape[0], dtype="float32") * (shape[-1] * 0.5)
+    elif numpy.array(centre).size == 1:
+        centre = numpy.ones(shape[0], dtype="float32") * centre
+    return numpy.asarray(centre).astype(dtype="float32", copy=False)
+
+
+def compute_angular_weights(angles):
+    """Angular weights (reconstruct.py:55-105): for each tilt in sorted order,
+    the mean of its neighbouring intervals weighted 2 for the adjacent ones
+    and 2/3 for the next ones, normalised by the mean interval."""
+    angles = numpy.asarray(angles)
+    order = numpy.argsort(angles)
+    a = angles[order]
+    n = a.size
+    if n > 1:
+        mean_interval = abs(a[-1] - a[0]) / (n - 1)
+        assert mean_interval != 0
+    else:
+        mean_interval = 1.0
+    weights = numpy.zeros(n)
+    for i in range(n):
+        terms = []
+        if i - 2 >= 0:
+            terms.append((1.0 / 1.5, a[i - 1] - a[i - 2]))
+        if i - 1 >= 0:
+            terms.append((2.0, a[i] - a[i - 1]))
+        if i + 1 < n:
+            terms.append((2.0, a[i + 1] - a[i]))
+        if i + 2 < n:
+            terms.append((1.0 / 1.5, a[i + 2] - a[i + 1]))
+        sw = sum(w for w, _ in terms)
+        si = sum(w * d for w, d in terms)
+        weights[order[i]] = si / sw if sw != 0 else 1.0
+    return weights / mean_interval
+
+
+def get_weights(angles, angular_weights=False):
+    """Per-tilt sinogram weights (reconstruct.py:108-124)."""
+    if angular_weights:
+        return compute_angular_weights(angles)
+    return numpy.ones(numpy.asarray(angles).shape)
+
+
+def reconstruct(tomogram, angles, reconstruction=None, centre=None,
+                pixel_size=1, min_defocus=0, max_defocus=0,
+                sinogram_order=False, transform=None, angular_weights=False,
+                device="cpu", ncore=None, nchunk=None, gpu_list=None,
+                storage="f32", depth=None):
+    """Filtered backprojection of a tilt series (reconstruct.py:127-213).
+
+    tomogram: (T, Y, X) projections or (T, ndf, Y, X) defocus-corrected
+    projections, or the same in sinogram order (Y, T, X) / (Y, ndf, T, X)
+    with sinogram_order=True. Returns the (Y, X, X) volume (or writes
+    `reconstruction`). Extensions: `storage` ("f32", "f16", "bf16") selects
+    the filtered-projection precision on the GPU; `depth` reconstructs only
+    the central `depth` rows of each square slice, shape (Y, depth, X).
+    """
+    assert len(tomogram.shape) in [3, 4], "Dimensions must be 3 or 4"
+    sinogram = tomogram.astype(dtype="float32", copy=False)
+    if not sinogram_order:
+        sinogram = numpy.swapaxes(sinogram, 0, -2)  # a view, as the reference
+    X = sinogram.shape[-1]
+    if reconstruction is None:
+        shape = (sinogram.shape[0], X if depth is None else depth, X)
+        reconstruction = numpy.zeros(shape, dtype="float32")
+    else:
+        assert reconstruction.shape[0] == sinogram.shape[0], \
+            "Inconsistent reconstruction/sinogram shape"
+        assert reconstruction.shape[2] == X, \
+            "Inconsistent reconstruction/sinogram shape"
+        if depth is None:
+            assert reconstruction.shape[1] == X, \
+                "Inconsistent reconstruction/sinogram shape"
+    centre = get_centre(sinogram.shape, centre)
+    weights = get_weights(angles, angular_weights)
+    mp.reconstruction_dispatcher(
+        sinogram, reconstruction, centre, angles, weights=weights,
+        pixel_size=pixel_size, min_defocus=min_defocus,
+        max_defocus=max_defocus, transform=transform, device=device,
+        ncore=ncore, nchunk=nchunk, gpu_list=gpu_list,
+        tomogram=tomogram, sinogram_order=sinogram_order, storage=storage)
+    return reconstruction
+
+
+def reconstruct_projections(projections, angles, centre=None, pixel_size=1,
+                            energy=300, defocus=None, num_defocus=None,
+                            step_defocus=None, spherical_aberration=0,
+                            astigmatism=0, astigmatism_angle=0,
+                            phase_shift=0, transform=None,
+                            angular_weights=False, depth=None,
+                            storage="f32", reconstruction=None,
+                            gpu_index=-1):
+    """CTF-corrected FBP of raw projections (T, Y, X) in one device call.
+
+    Equivalent to correct_projections(...) followed by reconstruct(...) with
+    the returned relative defocus range (reconstruct.py:341-388), with the
+    same defocus steps (correct.py:44-93) and CTF (correct.py:116-214); the
+    corrected stack stays on the device. defocus=None reconstructs without
+    correction. Units as the reference: energy keV, defoci A, Cs mm, Ca A,
+    Pa and phase_shift degrees.
+    """
+    projections = numpy.ascontiguousarray(projections, dtype=numpy.float32)
+    assert projections.ndim == 3, "projections must be (T, Y, X)"
+    T, Y, X = projections.shape
+    c = get_centre(projections.shape, centre, False)
+    assert numpy.all(c == c[0]), "one rotation centre for all rows"
+    weights = get_weights(angles, angular_weights).astype(numpy.float32)
+    if transform == "minus":
+        weights = -weights
+    shape = (Y, X if depth is None else depth, X)
+    out = reconstruction
+    if out is None:
+        out = numpy.zeros(shape, numpy.float32)
+    kw = {}
+    mn_rel = mx_rel = 0.0
+    if defocus is not None:
+        mn, mx, step, ndf = get_defocus_steps(defocus, num_defocus,
+                                              step_defocus, X, c, pixel_size)
+        assert abs(mn + (ndf - 1) * step - mx) < 1e-7, \
+            "Inconsistent defocus steps"
+        dfs = numpy.array([mn + d * step for d in range(ndf)],
+                          dtype=numpy.float64)
+        kw = dict(defocus=dfs,
+                  wavelength=ext.get_electron_wavelength(energy * 1000),
+                  Cs=spherical_aberration * 1e7, Ca=astigmatism,
+                  Pa=astigmatism_angle * numpy.pi / 180,
+                  phi=phase_shift * numpy.pi / 180)
+        mn_rel, mx_rel = mn - defocus, mx - defocus
+    ext.reconstruct_volume(projections, out,
+                           numpy.asarray(angles, numpy.float32), 0,
+                           float(c[0]), pixel_size, mn_rel, mx_rel,
+                           weights=weights, storage=storage,
+                           gpu_index=gpu_index, **kw)
+    return out

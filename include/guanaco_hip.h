@@ -171,6 +171,13 @@ int gx_volume_run(gx_volume_plan *plan, const float *d_in, float *d_vol,
  * and backprojection of the slabs. */
 int gx_volume_filter(gx_volume_plan *plan, const float *d_in, void *stream);
 int gx_volume_backproject(gx_volume_plan *plan, float *d_vol, void *stream);
+/* Backprojection of detector rows [y0, y1) only, into d_vol holding those
+ * rows ((y1 - y0) x Z x W). y0 must be a multiple of the plan's rows per
+ * block, y1 too unless it is num_rows: volumes larger than the free HBM are
+ * written band by band. */
+int gx_volume_backproject_rows(gx_volume_plan *plan, size_t y0, size_t y1,
+                               float *d_vol, void *stream);
+int gx_volume_rows_per_block(gx_volume_plan *plan, size_t *rows);
 /* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage. */
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);
