@@ -37,6 +37,8 @@ METRIC = ("voxel·tilt updates/s at ≤1e-3 rel-L2 vs FP64, batched vs "
           "plain; p50/p99 latency")
 UNIT = "voxel·tilt updates/s"
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 128 B/clk x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
 
 # name: (X, Y, T, tilt step deg, Z, ndf, pixel A, defocus A, phi rad,
 #        ellipsoids, semi-axes px, noise, seed, description)
@@ -270,6 +272,9 @@ def main():
     ap.add_argument("--dump", default=None,
                     help="directory to save the arrays the oracle check "
                          "compares")
+    ap.add_argument("--bp", choices=["staged", "direct"], default=None,
+                    help="backprojection kernel (default: the plan's "
+                         "choice, staged where a tile spans <= 4 slabs)")
     ap.add_argument("--only", choices=["plain", "batched"], default=None,
                     help="time one path only")
     ap.add_argument("--profile-kernels", action="store_true",
@@ -321,6 +326,8 @@ def main():
                         (("f32",) if args.only == "plain" else
                          (args.storage,))):
             plan = make_plans(C, w, angles, storage)
+            if args.bp:
+                plan.set_backprojector(args.bp)
             plan.filter(d_in, stream)
             plan.backproject(d_vol, stream)
             plan.backproject(d_vol, stream)
@@ -344,12 +351,22 @@ def main():
         if args.only and args.only != path:
             continue
         plan = make_plans(C, w, angles, storage)
+        if args.bp:
+            plan.set_backprojector(args.bp)
+        kind, slots = plan.backprojector()
         plans[path] = plan
         ms, lat, wall = time_path(C, plan, d_in, vol, stream, args.steps,
                                   args.warmup, graph)
         filt_ms = time_kernel(C, lambda: plan.filter(d_in, stream), stream, 3)
         bp_ms = time_kernel(C, lambda: plan.backproject(vol, stream), stream,
                             max(3, args.steps))
+        other = {}
+        if kind == C.BP_STAGED:
+            # the direct kernel on the same slabs, for comparison
+            plan.set_backprojector("direct")
+            other["bp_direct_ms"] = time_kernel(
+                C, lambda: plan.backproject(vol, stream), stream, 3)
+            plan.set_backprojector("staged")
         plan.run(d_in, vol, stream)
         stream.synchronize()
         results[path] = dict(storage=storage, ms_per_step=ms,
@@ -357,7 +374,10 @@ def main():
                              p50_ms=percentile(lat, 50),
                              p99_ms=percentile(lat, 99), wall_s=wall,
                              filter_ms=filt_ms, bp_ms=bp_ms,
-                             bp_updates_per_s=updates / (bp_ms * 1e-3))
+                             bp_updates_per_s=updates / (bp_ms * 1e-3),
+                             backprojector=("staged, %d slab(s) per tilt"
+                                            % slots if kind == C.BP_STAGED
+                                            else "direct"), **other)
         print("bench: %s (%s) %.2f ms/step, BP %.2f ms, filter %.2f ms"
               % (path, storage, ms, bp_ms, filt_ms), file=sys.stderr,
               flush=True)
@@ -407,6 +427,12 @@ def main():
             "algorithmic_bytes_per_update": b_alg,
             "updates_per_launch": updates,
             "launch_ms": head["bp_ms"],
+            # taps read from LDS by the staged kernel: 2 * s bytes per update
+            "lds_read": {
+                "achieved": updates * 2 * s_bytes / (head["bp_ms"] * 1e-3)
+                / 1e9,
+                "peak": LDS_PEAK_GBS, "unit": "GB/s",
+            } if head["backprojector"].startswith("staged") else None,
         },
     }
     if "batched" in results and "plain" in results:

@@ -21,6 +21,9 @@ STORE_F32 = 0
 STORE_F16 = 1
 STORE_BF16 = 2
 STORAGE = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16}
+BP_DIRECT = 0
+BP_STAGED = 1
+BACKPROJECTOR = {"direct": BP_DIRECT, "staged": BP_STAGED}
 
 INPUT_PROJECTIONS = 0
 INPUT_CORRECTED = 1
@@ -127,6 +130,9 @@ SIGNATURES = {
     "gx_volume_backproject": [_vp, _vp, _vp],
     "gx_volume_backproject_rows": [_vp, _sz, _sz, _vp, _vp],
     "gx_volume_rows_per_block": [_vp, ctypes.POINTER(_sz)],
+    "gx_volume_plan_set_backprojector": [_vp, _int],
+    "gx_volume_plan_get_backprojector": [_vp, ctypes.POINTER(_int),
+                                         ctypes.POINTER(_int)],
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
     "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],
@@ -333,6 +339,20 @@ class VolumePlan:
     def backproject(self, d_vol, stream=None):
         call("gx_volume_backproject", self.handle, d_vol.ptr,
              stream.handle if isinstance(stream, Stream) else stream)
+
+    def set_backprojector(self, kind):
+        """BP_DIRECT or BP_STAGED (the default where a tile spans <= 4
+        defocus slabs)."""
+        call("gx_volume_plan_set_backprojector", self.handle,
+             BACKPROJECTOR[kind] if isinstance(kind, str) else kind)
+
+    def backprojector(self):
+        """(kind, slabs held per tilt by the staged kernel; 0 = none)."""
+        k = _int()
+        n = _int()
+        call("gx_volume_plan_get_backprojector", self.handle, ctypes.byref(k),
+             ctypes.byref(n))
+        return k.value, n.value
 
     def slabs(self):
         p = _vp()

@@ -7,6 +7,7 @@
 // for whole volumes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -108,6 +109,20 @@ void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
   const float ds = ndf > 1 ? (float)ndf * ps / (mx - mn) : 0.0f;
   *dscale = ds;
   *doffset = -ds * (mn / ps);
+}
+
+// Slabs the LDS-staged backprojector must hold per tilt: a 64 x 4 voxel tile
+// spans at most 63.08 units of h (its diagonal), i.e. 63.08 |dscale| slab
+// widths, so at most ceil of that plus one consecutive slabs (63.2 leaves a
+// margin for rounding). Rounded up to a compiled form (1, 2 or 4); 0 when
+// more are needed and the direct kernel must run.
+int bp_slots(size_t ndf, float dscale) {
+  if (ndf <= 1) return 1;
+  const double span = 63.2 * std::fabs((double)dscale);
+  size_t need = ndf;
+  if (std::isfinite(span) && span < (double)ndf)
+    need = std::min(ndf, (size_t)std::ceil(span) + 1);
+  return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 0;
 }
 
 // Ramp filter spectrum for rows of N samples zero-padded to L, including the
@@ -573,6 +588,8 @@ struct gx_volume_plan {
   int X = 0, Y = 0, T = 0, ndf = 0, Z = 0, W = 0;
   int storage = 0, layout = 0;
   int YB = 0, NYB = 0, XB = 0, L = 0, chunk = 0;
+  int bp_kind = GX_BP_STAGED;
+  int bp_staged = 1;  // slabs per tilt of the staged kernel; 0 = direct
   bool ctf = false;
   float dscale = 0, doffset = 0, nrm_mul = 1;
   size_t slab_bytes = 0, total_bytes = 0;
@@ -594,6 +611,9 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
   GX_CHECK(d->pixel_size > 0);
   GX_CHECK(d->min_defocus <= d->max_defocus);
   GX_CHECK(d->angles != nullptr);
+  for (size_t t = 0; t < d->num_angles; ++t)
+    if (!std::isfinite(d->angles[t]))
+      return fail(GX_EINVAL, "angle %zu is not finite", t);
   GX_CHECK(d->storage >= GX_STORE_F32 && d->storage <= GX_STORE_BF16);
   GX_CHECK(d->input_layout >= GX_INPUT_PROJECTIONS &&
            d->input_layout <= GX_INPUT_SINOGRAMS);
@@ -688,6 +708,8 @@ extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
   p->nrm_mul = ps.ctf ? (float)d->num_pixels * (float)d->num_rows : 1.0f;
   defocus_coefficients(d->num_defocus, d->pixel_size, d->min_defocus,
                        d->max_defocus, &p->dscale, &p->doffset);
+  p->bp_staged = bp_slots(d->num_defocus, p->dscale);
+  p->bp_kind = p->bp_staged ? GX_BP_STAGED : GX_BP_DIRECT;
 
   const size_t T = d->num_angles;
   std::vector<float> coef(4 * T), w(T, 1.0f);
@@ -799,6 +821,7 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   a.yb1 = (int)((y1 + p->YB - 1) / p->YB);
   a.y_base = (int)y0;
   a.vol = d_vol;
+  a.staged = p->bp_kind == GX_BP_STAGED ? p->bp_staged : 0;
   GX_HIP(launch_bp(a, p->YB, p->storage, (hipStream_t)stream));
   return GX_OK;
 }
@@ -807,6 +830,25 @@ extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
                                      void *stream) {
   GX_CHECK(p != nullptr);
   return gx_volume_backproject_rows(p, 0, (size_t)p->Y, d_vol, stream);
+}
+
+extern "C" int gx_volume_plan_set_backprojector(gx_volume_plan *p, int kind) {
+  GX_CHECK(p != nullptr);
+  GX_CHECK(kind == GX_BP_DIRECT || kind == GX_BP_STAGED);
+  if (kind == GX_BP_STAGED && p->bp_staged == 0)
+    return fail(GX_EUNSUPPORTED,
+                "a voxel tile spans more than 4 defocus slabs; the staged "
+                "backprojector holds at most 4");
+  p->bp_kind = kind;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_get_backprojector(gx_volume_plan *p, int *kind,
+                                                int *slabs_per_tilt) {
+  GX_CHECK(p != nullptr);
+  if (kind) *kind = p->bp_kind;
+  if (slabs_per_tilt) *slabs_per_tilt = p->bp_staged;
+  return GX_OK;
 }
 
 extern "C" int gx_volume_rows_per_block(gx_volume_plan *p, size_t *rows) {

@@ -18,6 +18,8 @@ struct BpArgs {
   int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
   int y_base;                // vol holds rows [y_base, ...)
   float *vol;                // [y - y_base][z][x]
+  int staged;                // 0: direct kernel; 1, 2, 4: LDS-staged kernel
+                             // holding that many slabs per tilt
 };
 
 // Each returns hipSuccess or the launch error.

@@ -178,6 +178,17 @@ int gx_volume_backproject(gx_volume_plan *plan, float *d_vol, void *stream);
 int gx_volume_backproject_rows(gx_volume_plan *plan, size_t y0, size_t y1,
                                float *d_vol, void *stream);
 int gx_volume_rows_per_block(gx_volume_plan *plan, size_t *rows);
+/* Backprojection kernel of a plan (no reference counterpart; the reference
+ * has one CPU and one texture-based GPU kernel). GX_BP_STAGED, the default,
+ * copies each voxel tile's detector window into LDS per batch of tilts and
+ * holds 1, 2 or 4 defocus slabs per tilt; when a tile spans more slabs
+ * (narrow slabs, large |dscale|) plans use GX_BP_DIRECT, which gathers taps
+ * straight from global memory. Both give identical results. */
+#define GX_BP_DIRECT 0
+#define GX_BP_STAGED 1
+int gx_volume_plan_set_backprojector(gx_volume_plan *plan, int kind);
+int gx_volume_plan_get_backprojector(gx_volume_plan *plan, int *kind,
+                                     int *slabs_per_tilt);
 /* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage. */
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);

@@ -301,3 +301,92 @@ def test_identical_tilts_give_identical_slabs(capi, chunk):
             assert not bad, (storage, run, bad)
         s.destroy()
         plan.destroy()
+
+
+# ------------------------------------------ staged and direct backprojection
+
+BP_KIND_CASES = [
+    # X, Y, T, ndf, Z, W, (first, last) tilt deg, (min_df, max_df), pixel
+    (96, 20, 37, 1, 24, 70, (-60.0, 60.0), (0.0, 0.0), 1.0),
+    # every quadrant (negative cosines), T > 64: several tilt chunks
+    (64, 16, 150, 1, 13, 64, (-180.0, 178.0), (0.0, 0.0), 1.0),
+    # three slabs per tile at most -> four staged
+    (128, 24, 130, 5, 40, 128, (-70.0, 70.0), (-300.0, 300.0), 2.0),
+    # two slabs per tile at most
+    (64, 16, 21, 8, 64, 200, (-60.0, 60.0), (-512.0, 512.0), 1.0),
+    # narrow slabs: more than four per tile, direct kernel only
+    (64, 16, 21, 16, 64, 64, (-60.0, 60.0), (-32.0, 32.0), 1.0),
+]
+
+
+def sinogram_plan(case, storage):
+    X, Y, T, ndf, Z, W, (a0, a1), (mn, mx), ps = case
+    angles = numpy.deg2rad(numpy.linspace(a0, a1, T)).astype(numpy.float32)
+    rng = numpy.random.default_rng(X + T + ndf)
+    sino = rng.standard_normal((Y, ndf, T, X)).astype(numpy.float32)
+    plan = C.VolumePlan(X, Y, angles, Z, width=W, num_defocus=ndf,
+                        pixel_size=ps, centre=X / 2.0 - 0.7, min_defocus=mn,
+                        max_defocus=mx, storage=storage,
+                        input_layout=C.INPUT_SINOGRAMS)
+    return plan, sino, angles
+
+
+def slab_rows(plan, case, ys):
+    """The plan's filtered FP32 slabs, [k][t][y/YB][x][y%YB], as one
+    (ndf, T, X) array per detector row in ys."""
+    X, Y, T, ndf = case[:4]
+    p, n, yb = plan.slabs()
+    a = numpy.empty(n // 4, numpy.float32)
+    C.call("gx_memcpy_dtoh", a.ctypes.data, p, n, None)
+    a = a.reshape(ndf, T, -1, X, yb)
+    return {y: numpy.ascontiguousarray(a[:, :, y // yb, :, y % yb])
+            for y in ys}
+
+
+@pytest.mark.parametrize("case", BP_KIND_CASES)
+def test_staged_and_direct_bp_bitexact_vs_oracle(capi, case):
+    X, Y, T, ndf, Z, W, _, (mn, mx), ps = case
+    plan, sino, angles = sinogram_plan(case, "f32")
+    kind, slots = plan.backprojector()
+    assert (kind == C.BP_STAGED) == (slots > 0)
+    d_in = C.DeviceArray.from_host(sino)
+    plan.filter(d_in)
+    vols = {}
+    for k in ("direct", "staged"):
+        if k == "staged" and slots == 0:
+            with pytest.raises(C.GxError, match="more than 4 defocus slabs"):
+                plan.set_backprojector(k)
+            continue
+        plan.set_backprojector(k)
+        d_vol = C.DeviceArray((Y, Z, W))
+        C.call("gx_fill_u32", d_vol.ptr, 0x7fc00000, Y * Z * W, None)
+        plan.backproject(d_vol)
+        C.call("gx_device_synchronize")
+        vols[k] = d_vol.to_host()
+    ys = [0, Y // 2, Y - 1]
+    rows = slab_rows(plan, case, ys)
+    for y in ys:
+        s = rows[y] if ndf > 1 else rows[y][0]
+        expect = O.project(s, (Z, W), angles, X / 2.0 - 0.7, ps, mn, mx)
+        for k, v in vols.items():
+            numpy.testing.assert_array_equal(v[y], expect, err_msg=k)
+    plan.destroy()
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("case", BP_KIND_CASES[:4])
+def test_staged_bp_equals_direct_reduced_storage(capi, case, storage):
+    X, Y, T, ndf, Z, W = case[:6]
+    plan, sino, angles = sinogram_plan(case, storage)
+    d_in = C.DeviceArray.from_host(sino)
+    plan.filter(d_in)
+    out = {}
+    for k in ("direct", "staged"):
+        plan.set_backprojector(k)
+        d_vol = C.DeviceArray((Y, Z, W))
+        plan.backproject(d_vol)
+        C.call("gx_device_synchronize")
+        out[k] = d_vol.to_host()
+    assert numpy.isfinite(out["staged"]).all()
+    numpy.testing.assert_array_equal(out["staged"], out["direct"])
+    plan.destroy()
