@@ -771,7 +771,7 @@ extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
                          p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s));
       GX_HIP(launch_rows_c2r_filter(
           p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
-          p->ramp.as<float>(), p->wscale.as<float>(), tmax, p->nrm_mul,
+          p->wscale.as<float>(), tmax, p->nrm_mul,
           p->storage, p->slab.p, p->NYB, p->YB, s));
     }
   } else {

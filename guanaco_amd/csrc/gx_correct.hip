@@ -161,7 +161,7 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
     const float2 *__restrict__ tw) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
-  __shared__ float2 s[2][X];
+  __shared__ float2 s[2][fft_lds_len(X, E)];
   __shared__ float red[16];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int t = blockIdx.y;
@@ -217,19 +217,22 @@ __device__ __forceinline__ float mask_value(M m) {
 }
 
 template <int Y, typename M>
-__global__ __launch_bounds__(CB *(Y / default_e(Y))) void k_cols(
+__global__ __launch_bounds__(CB *(Y / default_e(Y)))
+__attribute__((amdgpu_waves_per_eu(4))) void k_cols(
     const float2 *__restrict__ F, const M *__restrict__ mask,
     float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw) {
   constexpr int E = default_e(Y);
   constexpr int NT = Y / E;
-  __shared__ float2 s[Y * CB];
+  __shared__ float2 s[fft_lds_len(Y, E) * CB];
   const int ci = threadIdx.x % CB, tid = threadIdx.x / CB;
   const int cb = blockIdx.x, t = blockIdx.y;
   const float2 *src = F + ((size_t)t * XB + cb) * Y * CB;
+  typename Fft<Y, E, -1, CB>::Twiddles twy;  // shared by every transform here
+  twy.load(tid, tw);
   float2 v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[(size_t)(tid + m * NT) * CB + ci];
-  Fft<Y, E, -1, CB>::run(v, s + ci, tid, tw);
+  Fft<Y, E, -1, CB>::run(v, s + ci, tid, twy);
   for (int k = 0; k < ndf; ++k) {
     const M *mk = mask + ((size_t)k * XB + cb) * Y * CB;
     float2 w[E];
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(CB *(Y / default_e(Y))) void k_cols(
       const float f = mask_value(mk[(size_t)(tid + m * NT) * CB + ci]);
       w[m] = make_float2(v[m].x * f, v[m].y * f);
     }
-    Fft<Y, E, 1, CB>::run(w, s + ci, tid, tw);
+    Fft<Y, E, 1, CB>::run(w, s + ci, tid, twy);
     float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[(size_t)(tid + m * NT) * CB + ci] = w[m];
@@ -274,8 +277,7 @@ __device__ __forceinline__ float inv_norm(const uint32_t *tmax, int t,
 template <int X, int STORE>
 __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
     k_rows_c2r_filter(const float2 *__restrict__ G, int Y, int XB, int ndf,
-                      int t0, int T, const float *__restrict__ ramp,
-                      const float *__restrict__ wscale,
+                      int t0, int T, const float *__restrict__ wscale,
                       const uint32_t *__restrict__ tmax, float nrm_mul,
                       typename StoreT<STORE>::T *__restrict__ slab, int NYB,
                       int YB, const float2 *__restrict__ tw) {
@@ -284,12 +286,16 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   constexpr int EX = EL / 2;
   constexpr int NT = L / EL;
   static_assert(X / EX == NT, "thread layouts must agree");
-  __shared__ float2 s[2][L];
+  __shared__ float2 s[2][fft_lds_len(L, EL)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
   const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
 
+  // the L-point twiddles serve both L-point transforms; loaded first so that
+  // their latency hides behind the spectrum loads and the X-point inverse
+  typename Fft<L, EL, -1, 1>::Twiddles twl;
+  twl.load(tid, tw);
   float2 v[EX];
 #pragma unroll
   for (int m = 0; m < EX; ++m)
@@ -303,14 +309,18 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   for (int m = 0; m < EL; ++m)
     u[m] = m < EL / 2 ? s[p][tid + m * NT] : make_float2(0.0f, 0.0f);
   lds_barrier();
-  Fft<L, EL, -1, 1>::run(u, s[p], tid, tw);
+  Fft<L, EL, -1, 1>::run(u, s[p], tid, twl);
+  // filter.cpp:30-45: |j| / X on the L = 2X grid, times the 1/L of the
+  // unnormalised inverse; X and L are powers of two, so j / (X L) is exact
+  // and equals the reference's (j / X) * (1 / L)
+  constexpr float kRamp = 1.0f / ((float)X * (float)L);
 #pragma unroll
   for (int m = 0; m < EL; ++m) {
     const int j = tid + m * NT;
-    const float r = ramp[j <= X ? j : L - j];
+    const float r = (float)(j <= X ? j : L - j) * kRamp;
     u[m] = make_float2(u[m].x * r, u[m].y * r);
   }
-  Fft<L, EL, 1, 1>::run(u, s[p], tid, tw);
+  Fft<L, EL, 1, 1>::run(u, s[p], tid, twl);
   const float sc = wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
   using ST = StoreT<STORE>;
 #pragma unroll
@@ -328,7 +338,7 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_c2r(
     float *__restrict__ out, const float2 *__restrict__ tw) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
-  __shared__ float2 s[2][X];
+  __shared__ float2 s[2][fft_lds_len(X, E)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z;
   const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
@@ -361,7 +371,7 @@ __global__ __launch_bounds__(2 * (L / default_e(L))) void k_filter_rows(
     int YB, const float2 *__restrict__ tw) {
   constexpr int EL = default_e(L);
   constexpr int NT = L / EL;
-  __shared__ float2 s[2][L];
+  __shared__ float2 s[2][fft_lds_len(L, EL)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const long nrows = (long)ndf * T * Y;
   const long r0 = ((long)blockIdx.x * 2 + p) * 2, r1 = r0 + 1;
@@ -483,7 +493,7 @@ hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
 template <int STORE>
 static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       int ndf, int t0, int ntilts, int T,
-                                      const float *ramp, const float *wscale,
+                                      const float *wscale,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
                                       hipStream_t s, const float2 *tw) {
@@ -494,8 +504,8 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
   case n:                                                                      \
     hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE>), grid,                    \
                        dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
-                       ndf, t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab,    \
-                       NYB, YB, tw);                                           \
+                       ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
+                       tw);                                                    \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
     C_(4096)
@@ -508,7 +518,7 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
 
 hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   int ndf, int t0, int ntilts, int T,
-                                  const float *ramp, const float *wscale,
+                                  const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
                                   hipStream_t s) {
@@ -517,16 +527,16 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
   if (err != hipSuccess) return err;
   switch (store) {
     case kF32:
-      return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T, ramp,
+      return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
                                        s, tw);
     case kF16:
-      return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T, ramp,
+      return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
                                        s, tw);
     case kBF16:
       return launch_c2r_filter_t<kBF16>(G, X, Y, XB, ndf, t0, ntilts, T,
-                                        ramp, wscale, tmax, nrm_mul, slab,
+                                        wscale, tmax, nrm_mul, slab,
                                         NYB, YB, s, tw);
   }
   return hipErrorInvalidValue;

@@ -9,7 +9,11 @@
 // exchange through LDS. Several transforms can share a workgroup; each uses
 // its own LDS region, addressed as lds[i * LS] so that columns can be
 // interleaved. Every thread of the workgroup must call run() (it contains
-// workgroup barriers).
+// workgroup barriers), unless WAVE is set: then one transform is computed by
+// exactly one wave (N / E == 64) in its own LDS region and the exchanges are
+// ordered with wave-scope syncs only (a wave's LDS operations complete in
+// issue order), so the waves of a workgroup run their transforms
+// independently.
 //
 // Conventions follow FFTW (the reference's host FFT, src/libguanaco/
 // fft.cpp:38-71): DIR = -1 is exp(-2 pi i jk/N), DIR = +1 is exp(+...), both
@@ -25,6 +29,20 @@ namespace gx {
 
 constexpr int kTwN = 16384;  // twiddle table length: largest supported N
 
+// LDS slot of element i of a transform computed with E values per thread:
+// one float2 of padding after every E, so that the first stage's stores
+// (lane tid writes elements tid * E .. tid * E + E - 1) and the strided
+// accesses of the later stages fall in different banks. Every stride the
+// stages use (E, N/E, N/R, and the products of earlier radices) is a multiple
+// of E, so slot(i + c) = slot(i) + c + c / E: a per-thread base plus constant
+// offsets. A transform's region holds fft_lds_len(N, E) * LS float2.
+__host__ __device__ constexpr int fft_lds_len(int n, int e) {
+  return n + n / e;
+}
+
+// The transforms may fuse multiply-adds (FP32 rounding of order 1e-7): only
+// the backprojector reproduces the reference's operation order exactly, and
+// it does not use these helpers.
 __host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) {
   return make_float2(a.x + b.x, a.y + b.y);
 }
@@ -32,6 +50,7 @@ __host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) {
   return make_float2(a.x - b.x, a.y - b.y);
 }
 __host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  #pragma clang fp contract(fast)
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // multiply by DIR * i
@@ -85,6 +104,7 @@ __host__ __device__ __forceinline__ float2 rot16(float2 a) {
 
 template <int R, int DIR>
 __host__ __device__ __forceinline__ void dft(float2 *v) {
+  #pragma clang fp contract(fast)
   if constexpr (R == 1) {
     return;
   } else if constexpr (R == 2) {
@@ -187,69 +207,168 @@ __host__ __device__ constexpr int default_e(int N) {
   return N >= 256 ? 16 : (N >= 64 ? 8 : (N >= 16 ? 4 : (N >= 4 ? 2 : N)));
 }
 
-template <int N, int E, int DIR, int LS>
+// Orders the LDS accesses of one wave before its later ones: no instruction
+// is needed in hardware; the fences and the empty asm keep the compiler from
+// moving LDS accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
+}
+
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+  if constexpr (WAVE) {
+    wave_lds_sync();
+  } else {
+    lds_barrier();
+  }
+}
+
+// Per-thread twiddle factors of one N-point transform with E values per
+// thread (raw table values W^j = exp(-2 pi i j / kTwN); a transform with
+// DIR = +1 conjugates them). Loaded once, before the first stage, so that
+// their latency overlaps the first radix pass, and reusable by every
+// transform of that size the thread computes.
+//
+// Middle stage S multiplies element m of each group by W^(m k kTwN/(Ns R)),
+// k = b mod Ns. Every b = tid + q N/E of a thread has the same k (Ns divides
+// N/E), so a stage needs R - 1 values. The last stage needs W^(m tid kTwN/N)
+// for m < E: formed as lo[m mod 4] * hi[m / 4] (one rounding) from at most
+// three values of each kind.
+template <int N, int E>
+struct FftTwiddles {
+  static constexpr int NS = plan_stages(N, E);
+  static constexpr int mid_before(int S) {
+    int c = 0;
+    for (int s = 1; s < S && s < NS - 1; ++s) c += plan_radix(N, E, s) - 1;
+    return c;
+  }
+  static constexpr int kMid = mid_before(NS);
+  static constexpr int kLo = E - 1 < 3 ? E - 1 : 3;      // W^(m1 tid), m1 = 1..3
+  static constexpr int kHi = E / 4 > 1 ? E / 4 - 1 : 0;  // W^(4 m2 tid)
+  float2 mid[kMid > 0 ? kMid : 1];
+  float2 lo[kLo > 0 ? kLo : 1];
+  float2 hi[kHi > 0 ? kHi : 1];
+
+  template <int S>
+  __device__ __forceinline__ void load_mid(int tid,
+                                           const float2 *__restrict__ tw) {
+    if constexpr (S < NS - 1) {
+      constexpr int R = plan_radix(N, E, S);
+      constexpr int Ns = plan_ns(N, E, S);
+      constexpr int TWS = kTwN / (Ns * R);
+      const int k = tid & (Ns - 1);
+#pragma unroll
+      for (int m = 1; m < R; ++m) mid[mid_before(S) + m - 1] = tw[m * k * TWS];
+      load_mid<S + 1>(tid, tw);
+    }
+  }
+  __device__ __forceinline__ void load(int tid, const float2 *__restrict__ tw) {
+    if constexpr (NS > 1) {
+      load_mid<1>(tid, tw);
+      constexpr int TWS = kTwN / N;
+#pragma unroll
+      for (int m = 1; m <= kLo; ++m) lo[m - 1] = tw[m * tid * TWS];
+#pragma unroll
+      for (int m = 1; m <= kHi; ++m) hi[m - 1] = tw[4 * m * tid * TWS];
+    }
+  }
+  // last-stage factor for element m (0 < m < E)
+  __device__ __forceinline__ float2 last(int m) const {
+    #pragma clang fp contract(fast)
+    const int m1 = m & 3, m2 = m >> 2;
+    if (m2 == 0) return lo[m1 - 1];
+    if (m1 == 0) return hi[m2 - 1];
+    return cmul(lo[m1 - 1], hi[m2 - 1]);
+  }
+};
+
+template <int DIR>
+__device__ __forceinline__ float2 dir_conj(float2 w) {
+  return DIR < 0 ? w : make_float2(w.x, -w.y);
+}
+
+template <int N, int E, int DIR, int LS, bool WAVE = false>
 struct Fft {
   static_assert((N & (N - 1)) == 0, "N must be a power of two");
   static_assert(N <= kTwN, "N exceeds the twiddle table");
   static constexpr int NT = N / E;  // threads per transform
+  static_assert(!WAVE || NT == 64, "a wave-scope transform is one wave");
   static constexpr int NS = plan_stages(N, E);
   static_assert(NS == 1 || N >= E * E, "invalid end radix");
+  static constexpr int kLds = fft_lds_len(N, E);
+  using Twiddles = FftTwiddles<N, E>;
+  // padded slot of element i, and of a constant offset c (a multiple of E)
+  __device__ __forceinline__ static int slot(int i) { return i + i / E; }
+  static constexpr int off(int c) { return c + c / E; }
 
   template <int S>
   __device__ __forceinline__ static void middle(float2 *lds, int tid,
-                                                const float2 *tw) {
+                                                const Twiddles &tw) {
+    #pragma clang fp contract(fast)
     if constexpr (S < NS - 1) {
       constexpr int R = plan_radix(N, E, S);
       constexpr int Ns = plan_ns(N, E, S);
       constexpr int NB = N / R;
       constexpr int Q = E / R;
-      constexpr int TWS = kTwN / (Ns * R);
+      constexpr int M0 = Twiddles::mid_before(S);
       float2 w[E];
+      const int sb = slot(tid);
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
-        const int b = tid + q * NT;
 #pragma unroll
-        for (int m = 0; m < R; ++m) w[q * R + m] = lds[(b + m * NB) * LS];
+        for (int m = 0; m < R; ++m)
+          w[q * R + m] = lds[(sb + off(q * NT + m * NB)) * LS];
       }
-      lds_barrier();
+      fft_sync<WAVE>();
+      const int k = tid & (Ns - 1);  // the same for every b = tid + q NT
 #pragma unroll
       for (int q = 0; q < Q; ++q) {
         const int b = tid + q * NT;
-        const int k = b & (Ns - 1);
 #pragma unroll
         for (int m = 1; m < R; ++m)
-          w[q * R + m] = cmul(w[q * R + m], twiddle<DIR>(tw, m * k * TWS));
+          w[q * R + m] = cmul(w[q * R + m], dir_conj<DIR>(tw.mid[M0 + m - 1]));
         dft<R, DIR>(&w[q * R]);
-        const int base = (b - k) * R + k;
+        const int sbase = slot((b - k) * R + k);
 #pragma unroll
-        for (int m = 0; m < R; ++m) lds[(base + m * Ns) * LS] = w[q * R + m];
+        for (int m = 0; m < R; ++m) lds[(sbase + off(m * Ns)) * LS] = w[q * R + m];
       }
-      lds_barrier();
+      fft_sync<WAVE>();
       middle<S + 1>(lds, tid, tw);
     }
   }
 
-  // In: v[m] = x[tid + m*NT]. Out: v[m] = X[tid + m*NT]. lds holds N*LS
+  // In: v[m] = x[tid + m*NT]. Out: v[m] = X[tid + m*NT]. lds holds kLds * LS
   // float2 (only every LS-th used). Returns with lds free for reuse.
   __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds,
-                                             int tid, const float2 *tw) {
+                                             int tid, const Twiddles &tw) {
+    #pragma clang fp contract(fast)
     if constexpr (NS == 1) {
       dft<E, DIR>(v);
     } else {
       dft<E, DIR>(v);
 #pragma unroll
-      for (int m = 0; m < E; ++m) lds[(tid * E + m) * LS] = v[m];
-      lds_barrier();
+      for (int m = 0; m < E; ++m) lds[(tid * (E + 1) + m) * LS] = v[m];
+      fft_sync<WAVE>();
       middle<1>(lds, tid, tw);
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[m] = lds[(tid + m * NT) * LS];
-      lds_barrier();
-      constexpr int TWS = kTwN / N;
+      for (int m = 0; m < E; ++m) v[m] = lds[(slot(tid) + off(m * NT)) * LS];
+      fft_sync<WAVE>();
 #pragma unroll
-      for (int m = 1; m < E; ++m)
-        v[m] = cmul(v[m], twiddle<DIR>(tw, m * tid * TWS));
+      for (int m = 1; m < E; ++m) v[m] = cmul(v[m], dir_conj<DIR>(tw.last(m)));
       dft<E, DIR>(v);
     }
+  }
+  // Same, loading this thread's twiddles first.
+  __device__ __forceinline__ static void run(float2 (&v)[E], float2 *lds,
+                                             int tid,
+                                             const float2 *__restrict__ tw) {
+    Twiddles t;
+    t.load(tid, tw);
+    run(v, lds, tid, t);
   }
 };
 

@@ -45,7 +45,7 @@ hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
                                 hipStream_t s);
 hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   int ndf, int t0, int ntilts, int T,
-                                  const float *ramp, const float *wscale,
+                                  const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
                                   hipStream_t s);
