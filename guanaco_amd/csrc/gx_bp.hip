@@ -10,21 +10,37 @@
 // each thread owns one (x, z) and YB consecutive rows: the coordinate work is
 // paid once per (voxel column, tilt) and the taps of all YB rows are one
 // contiguous 2*YB-value read from the [k][t][y/YB][x][y%YB] layout. A
-// workgroup is a 64 (x) by 4 (z) tile; consecutive logical workgroups are
+// workgroup is a 32 (x) by 8 (z) tile; consecutive logical workgroups are
 // z-neighbours and are kept on one XCD so their overlapping row segments hit
 // the same L2. The volume is written once, [y][z][x] with x fastest.
 //
 // kF32 reproduces the reference's FP32 operation order exactly (compiled
 // without FMA contraction): per-tilt constants computed on the host as the
 // reference does, tilt-outer accumulation order, no reassociation. kF16 and
-// kBF16 keep the same coordinate decisions, fold the normalisation into the
-// per-tilt scale and accumulate with FP32 fused multiply-adds.
+// kBF16 (whose error budget is set by the storage) compute p with two fused
+// multiply-adds from a table holding offset - 1/2 and the scale times the
+// normalisation the filter pass divided out, and accumulate with FP32 fused
+// multiply-adds. The per-tilt tables are read with scalar loads.
 #include "gx_common.hpp"
 #include "gx_kernels.hpp"
 
 namespace gx {
 
-constexpr int kTX = 64, kTZ = 4;
+constexpr int kTX = 32, kTZ = 8;
+
+// Per-tilt tables are read through the constant address space: uniform
+// indices become scalar loads, independent of the workgroup barriers.
+typedef const __attribute__((address_space(4))) float kfloat;
+typedef const __attribute__((address_space(4))) int kint;
+
+__device__ __forceinline__ float4 kload(const float4 *p, size_t i) {
+  kfloat *q = (kfloat *)p + 4 * i;
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int2 kload(const int2 *p, size_t i) {
+  kint *q = (kint *)p + 2 * i;
+  return make_int2(q[0], q[1]);
+}
 
 template <int YB, int STORE>
 struct Taps;
@@ -32,12 +48,13 @@ struct Taps;
 template <int YB>
 struct Taps<YB, kF32> {
   float v0[YB], v1[YB];
-  // The 2 * YB values as YB / 2 16-byte words (global memory or LDS).
-  __device__ __forceinline__ void load16(const uint4 *q) {
+  // The YB values of each tap as YB / 4 16-byte words at q0 and at q1
+  // (global memory or LDS).
+  __device__ __forceinline__ void load16(const uint4 *q0, const uint4 *q1) {
     static_assert(YB % 4 == 0, "16-byte taps need YB % 4 == 0");
 #pragma unroll
     for (int i = 0; i < YB / 4; ++i) {
-      const uint4 a = q[i], b = q[YB / 4 + i];
+      const uint4 a = q0[i], b = q1[i];
       v0[4 * i] = __uint_as_float(a.x); v0[4 * i + 1] = __uint_as_float(a.y);
       v0[4 * i + 2] = __uint_as_float(a.z); v0[4 * i + 3] = __uint_as_float(a.w);
       v1[4 * i] = __uint_as_float(b.x); v1[4 * i + 1] = __uint_as_float(b.y);
@@ -46,13 +63,8 @@ struct Taps<YB, kF32> {
   }
   __device__ __forceinline__ void load(const float *p) {
     if constexpr (YB % 4 == 0) {
-      const float4 *q = (const float4 *)__builtin_assume_aligned(p, 16);
-#pragma unroll
-      for (int i = 0; i < YB / 4; ++i) {
-        const float4 a = q[i], b = q[YB / 4 + i];
-        v0[4 * i] = a.x; v0[4 * i + 1] = a.y; v0[4 * i + 2] = a.z; v0[4 * i + 3] = a.w;
-        v1[4 * i] = b.x; v1[4 * i + 1] = b.y; v1[4 * i + 2] = b.z; v1[4 * i + 3] = b.w;
-      }
+      const uint4 *q = (const uint4 *)__builtin_assume_aligned(p, 16);
+      load16(q, q + YB / 4);
     } else {
 #pragma unroll
       for (int i = 0; i < YB; ++i) {
@@ -69,12 +81,13 @@ struct Taps {
   static_assert(YB % 8 == 0, "reduced storage needs YB % 8 == 0");
   float v0[YB], v1[YB];
   __device__ __forceinline__ void load(const T *p) {
-    load16((const uint4 *)__builtin_assume_aligned(p, 16));
+    const uint4 *q = (const uint4 *)__builtin_assume_aligned(p, 16);
+    load16(q, q + YB / 8);
   }
-  __device__ __forceinline__ void load16(const uint4 *q) {
+  __device__ __forceinline__ void load16(const uint4 *q0, const uint4 *q1) {
 #pragma unroll
     for (int i = 0; i < YB / 8; ++i) {
-      const uint4 a = q[i], b = q[YB / 8 + i];
+      const uint4 a = q0[i], b = q1[i];
       const uint32_t wa[4] = {a.x, a.y, a.z, a.w};
       const uint32_t wb[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
@@ -108,11 +121,19 @@ __device__ __forceinline__ float voxel_coord(int i, int n) {
 }
 
 // reconstructor.cpp:59-60,76-79: detector position p = x c - z s + offset,
-// i0 = floor(p - 0.5), t = (p - 0.5) - i0.
+// i0 = floor(p - 0.5), t = (p - 0.5) - i0. EXACT: the reference's operation
+// order; otherwise cf.z already holds offset - 1/2 and p is two fused
+// multiply-adds.
+template <bool EXACT>
 __device__ __forceinline__ int tap_index(float x, float z, float4 cf,
                                          float &tt) {
-  float p = x * cf.x - z * cf.y + cf.z;
-  p = p - 0.5f;
+  float p;
+  if constexpr (EXACT) {
+    p = x * cf.x - z * cf.y + cf.z;
+    p = p - 0.5f;
+  } else {
+    p = fmaf(x, cf.x, fmaf(-z, cf.y, cf.z));
+  }
   const float fi = floorf(p);
   tt = p - fi;
   return (int)fi;
@@ -130,22 +151,181 @@ __device__ __forceinline__ int slab_of(float x, float z, float4 cf,
   return min(max((int)n, 0), ndf - 1);
 }
 
-// Per-tilt multiplier of the reduced-storage forms: the reference's scale
-// times the normalisation the filter pass divided out.
-__device__ __forceinline__ float reduced_scale(float w, const uint32_t *tmax,
-                                               int t, float nrm_mul) {
-  float nrm = __uint_as_float(tmax[t]) * nrm_mul;
-  if (!(nrm > 0.0f)) nrm = 1.0f;
-  return w * nrm;
+// ------------------------------------------------------- per-tilt tables --
+
+// out[t] = (c, s, offset - 1/2, scale * normalisation): the normalisation
+// float(tmax[t]) * nrm_mul is what the filter pass divided the stored values
+// by (1 where it is not positive, or without tmax). Entries past T are zero.
+__global__ void k_bp_coefficients(const float4 *__restrict__ coef,
+                                  const uint32_t *__restrict__ tmax,
+                                  float nrm_mul, int T, int Tpad,
+                                  float4 *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= Tpad) return;
+  float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (t < T) {
+    const float4 cf = coef[t];
+    float nrm = tmax != nullptr ? __uint_as_float(tmax[t]) * nrm_mul : 1.0f;
+    if (!(nrm > 0.0f)) nrm = 1.0f;
+    r = make_float4(cf.x, cf.y, cf.z - 0.5f, cf.w * nrm);
+  }
+  out[t] = r;
+}
+
+hipError_t launch_bp_coefficients(const float4 *coef, const uint32_t *tmax,
+                                  float nrm_mul, int T, float4 *out,
+                                  hipStream_t s) {
+  const int Tpad = bp_tilt_pad(T);
+  hipLaunchKernelGGL(k_bp_coefficients, dim3((Tpad + 63) / 64), dim3(64), 0, s,
+                     coef, tmax, nrm_mul, T, Tpad, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// LDS-staged form.
+//
+// p is a chain of correctly rounded float operations, each monotone in x and
+// in z, so over a 32 x 8 tile i0 is smallest at one of the four corners, and
+// the exact range of p is at most 31|c| + 7|s| <= 31.79 sqrt(c^2 + s^2)
+// (c^2 + s^2 = 1 / (cos^2 + sin^2) to rounding): the taps of one tilt lie in
+// a window of 34 consecutive positions; kWPOS leaves a margin for rounding.
+// Likewise h spans at most 31.79 units, so a tile touches at most
+// ceil(31.79 |dscale|) + 1 consecutive slabs; the host picks KS, the slabs
+// staged per tilt, no smaller than that. The window base and first slab of
+// every (tile, tilt) come from a table computed once per plan with the same
+// arithmetic as the kernel. For a batch of TB tilts the workgroup copies, per
+// tilt and staged slab, kWPOS positions of its row block starting at the
+// window base (one contiguous run of the slab, YB rows per position) into
+// LDS, then interpolates from LDS with the same arithmetic per voxel as the
+// direct kernel. LDS indices are clamped, so coefficients that break the
+// bound (non-finite angles, which the host rejects) can only give wrong
+// values, never an access outside the buffer.
+//
+// In LDS a position's YB values are followed by 16 bytes of padding: the
+// stride is an odd number of 16-byte words, so the 16-byte reads of lanes at
+// consecutive positions fall in distinct banks.
+//
+// Two LDS buffers: the global loads of batch b + 1 are issued before batch b
+// is interpolated and written to LDS after it, one barrier per batch.
+constexpr int kWPOS = 40;
+
+template <int YB, int STORE, int KS>
+struct Staging {
+  using T = typename StoreT<STORE>::T;
+  static constexpr int kPosBytes = YB * (int)sizeof(T);
+  static constexpr int kPosWords = kPosBytes / 16;      // 16-byte words
+  static constexpr int kPosStride = kPosWords + 1;      // in LDS, padded
+  static constexpr int kTB = 256 / (kPosBytes * KS);    // tilts per batch
+  static constexpr int kNSlot = kTB * KS;  // (tilt, slab) runs per batch
+  static constexpr int kSlotWords = kWPOS * kPosStride;
+  static constexpr int kBufWords = kNSlot * kSlotWords;
+  static constexpr int kChunks = kWPOS * kPosWords;  // words per run
+  static constexpr int kCPL = (kChunks + 63) / 64;   // words per lane per run
+  static constexpr int kSPW = kNSlot / (kTX * kTZ / 64);  // runs per wave
+  static_assert(kPosBytes % 16 == 0, "a position is whole 16-byte words");
+  static_assert(kTB >= 1 && kBpTiltPad % kTB == 0,
+                "batches must tile the padded tilt tables");
+  static_assert(kNSlot % (kTX * kTZ / 64) == 0, "runs split evenly on waves");
+};
+
+// Window of one tilt over the tile's active voxels [x0, x1] x [z0, z1]:
+// smallest i0 and smallest k over the four corners.
+template <bool EXACT, bool MULTI>
+__device__ __forceinline__ int2 tile_window(float4 cf, float x0, float x1,
+                                            float z0, float z1, float dscale,
+                                            float doffset, int ndf) {
+  int w = 0x7fffffff, k = 0x7fffffff;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float x = (c & 1) ? x1 : x0;
+    const float z = (c & 2) ? z1 : z0;
+    float tt;
+    w = min(w, tap_index<EXACT>(x, z, cf, tt));
+    if constexpr (MULTI) k = min(k, slab_of(x, z, cf, dscale, doffset, ndf));
+  }
+  return make_int2(w, MULTI ? k : 0);
+}
+
+// win[tile * Tpad + t], tile = bx * nbz + bz (the kernels' order).
+template <bool EXACT, bool MULTI>
+__global__ void k_bp_windows(const float4 *__restrict__ coef, int T, int Tpad,
+                             int W, int Z, int nbx, int nbz, float dscale,
+                             float doffset, int ndf, int2 *__restrict__ win) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)nbx * nbz * Tpad) return;
+  const int t = (int)(i % Tpad);
+  const int tile = (int)(i / Tpad);
+  const int bz = tile % nbz, bx = tile / nbz;
+  int2 r = make_int2(0, 0);
+  if (t < T) {
+    const float x0 = voxel_coord(bx * kTX, W);
+    const float x1 = voxel_coord(min(bx * kTX + kTX, W) - 1, W);
+    const float z0 = voxel_coord(bz * kTZ, Z);
+    const float z1 = voxel_coord(min(bz * kTZ + kTZ, Z) - 1, Z);
+    r = tile_window<EXACT, MULTI>(coef[t], x0, x1, z0, z1, dscale, doffset,
+                                  ndf);
+  }
+  win[i] = r;
+}
+
+size_t bp_window_entries(int W, int Z, int T) {
+  const size_t nbx = (W + kTX - 1) / kTX, nbz = (Z + kTZ - 1) / kTZ;
+  return nbx * nbz * (size_t)bp_tilt_pad(T);
+}
+
+hipError_t launch_bp_windows(const float4 *coef, int exact, int T, int W,
+                             int Z, float dscale, float doffset, int ndf,
+                             int2 *win, hipStream_t s) {
+  const int nbx = (W + kTX - 1) / kTX, nbz = (Z + kTZ - 1) / kTZ;
+  const size_t n = bp_window_entries(W, Z, T);
+  const dim3 g((unsigned)((n + 255) / 256)), b(256);
+  const int Tpad = bp_tilt_pad(T);
+  const bool multi = ndf > 1;
+  if (exact && multi)
+    hipLaunchKernelGGL((k_bp_windows<true, true>), g, b, 0, s, coef, T, Tpad,
+                       W, Z, nbx, nbz, dscale, doffset, ndf, win);
+  else if (exact)
+    hipLaunchKernelGGL((k_bp_windows<true, false>), g, b, 0, s, coef, T, Tpad,
+                       W, Z, nbx, nbz, dscale, doffset, ndf, win);
+  else if (multi)
+    hipLaunchKernelGGL((k_bp_windows<false, true>), g, b, 0, s, coef, T, Tpad,
+                       W, Z, nbx, nbz, dscale, doffset, ndf, win);
+  else
+    hipLaunchKernelGGL((k_bp_windows<false, false>), g, b, 0, s, coef, T, Tpad,
+                       W, Z, nbx, nbz, dscale, doffset, ndf, win);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ kernels -----
+
+// Accumulate one tilt's taps (reconstructor.cpp:82-84).
+template <int YB, int STORE>
+__device__ __forceinline__ void accumulate(float (&acc)[YB],
+                                           const Taps<YB, STORE> &tp,
+                                           float4 cf, float tt) {
+  if constexpr (STORE == kF32) {
+    const float w0 = 1.0f - tt;
+#pragma unroll
+    for (int r = 0; r < YB; ++r)
+      acc[r] = acc[r] + cf.w * (w0 * tp.v0[r] + tt * tp.v1[r]);
+  } else {
+    const float w1 = cf.w * tt;  // cf.w: scale x normalisation
+    const float w0 = cf.w - w1;
+#pragma unroll
+    for (int r = 0; r < YB; ++r) {
+      acc[r] = fmaf(w0, tp.v0[r], acc[r]);
+      acc[r] = fmaf(w1, tp.v1[r], acc[r]);
+    }
+  }
 }
 
 template <int YB, int STORE, bool MULTI>
 __global__ __launch_bounds__(kTX *kTZ) void k_bp(
     const typename StoreT<STORE>::T *__restrict__ slab,
-    const float4 *__restrict__ coef, const uint32_t *__restrict__ tmax,
-    float nrm_mul, float dscale, float doffset, int ndf, int N, int T,
-    int NYB, int W, int Z, int Y, int yb0, int y_base,
+    const float4 *__restrict__ coef, float dscale, float doffset, int ndf,
+    int N, int T, int NYB, int W, int Z, int Y, int yb0, int y_base,
     float *__restrict__ vol, int nbx, int nbz, unsigned nblocks) {
+  constexpr bool kExact = STORE == kF32;
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
   const int bz = (int)(lb % (unsigned)nbz);
   const int bx = (int)((lb / (unsigned)nbz) % (unsigned)nbx);
@@ -165,31 +345,16 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
   const typename StoreT<STORE>::T *yblock = slab + (size_t)byb * row_len;
 
   for (int t = 0; t < T; ++t) {
-    const float4 cf = coef[t];
+    const float4 cf = kload(coef, t);
     int k = 0;
     if constexpr (MULTI) k = slab_of(x, z, cf, dscale, doffset, ndf);
     float tt;
-    const int i0 = tap_index(x, z, cf, tt);
+    const int i0 = tap_index<kExact>(x, z, cf, tt);
     if (active && (unsigned)i0 < (unsigned)(N - 1)) {
-      const auto *src = yblock + (size_t)t * NYB * row_len +
-                        (size_t)k * kstride + (size_t)i0 * YB;
       Taps<YB, STORE> tp;
-      tp.load(src);
-      if constexpr (STORE == kF32) {
-        const float w0 = 1.0f - tt;
-#pragma unroll
-        for (int r = 0; r < YB; ++r)
-          acc[r] = acc[r] + cf.w * (w0 * tp.v0[r] + tt * tp.v1[r]);
-      } else {
-        const float sc = reduced_scale(cf.w, tmax, t, nrm_mul);
-        const float w1 = sc * tt;
-        const float w0 = sc - w1;
-#pragma unroll
-        for (int r = 0; r < YB; ++r) {
-          acc[r] = fmaf(w0, tp.v0[r], acc[r]);
-          acc[r] = fmaf(w1, tp.v1[r], acc[r]);
-        }
-      }
+      tp.load(yblock + (size_t)t * NYB * row_len + (size_t)k * kstride +
+              (size_t)i0 * YB);
+      accumulate<YB, STORE>(acc, tp, cf, tt);
     }
   }
   if (active) {
@@ -201,83 +366,18 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
   }
 }
 
-// ---------------------------------------------------------------------------
-// LDS-staged form.
-//
-// p is a chain of correctly rounded float operations, each monotone in x and
-// in z, so over a 64 x 4 tile i0 is smallest at one of the four corners, and
-// the exact range of p is at most 63|c| + 3|s| <= 63.08 sqrt(c^2 + s^2)
-// (c^2 + s^2 = 1 / (cos^2 + sin^2) to rounding): the taps of one tilt lie in
-// a window of 66 consecutive positions; kWPOS leaves a margin for rounding.
-// Likewise h spans at most 63.08 units, so a tile touches at most
-// ceil(63.08 |dscale|) + 1 consecutive slabs; the host picks KS, the slabs
-// staged per tilt, no smaller than that. For a batch of TB tilts the
-// workgroup copies, per tilt and staged slab, kWPOS positions of its row
-// block starting at the window base (one contiguous run of the slab, YB rows
-// per position) into LDS, then interpolates from LDS with the same
-// arithmetic per voxel as k_bp. LDS indices are clamped, so coefficients
-// that break the bound (non-finite angles, which the host rejects) can only
-// give wrong values, never an access outside the buffer.
-//
-// The coefficients, window bases and first staged slabs of 64 consecutive
-// tilts are computed by the 64 lanes of every wave (lane j for tilt j) and
-// read back with readlane. Two LDS buffers: the global loads of batch b + 1
-// are issued before batch b is interpolated and written to LDS after it, one
-// barrier per batch.
-constexpr int kWPOS = 72;
-
-template <int YB, int STORE, int KS>
-struct Staging {
-  using T = typename StoreT<STORE>::T;
-  static constexpr int kPosBytes = YB * (int)sizeof(T);
-  static constexpr int kTB = 256 / (kPosBytes * KS);  // tilts per batch
-  static constexpr int kNSlot = kTB * KS;  // (tilt, slab) runs per batch
-  static constexpr int kSlotBytes = kWPOS * kPosBytes;
-  static constexpr int kBufBytes = kNSlot * kSlotBytes;
-  static constexpr int kChunks = kSlotBytes / 16;  // 16-byte words per run
-  static constexpr int kCPL = (kChunks + 63) / 64; // words per lane per run
-  static constexpr int kSPW = kNSlot / (kTX * kTZ / 64);  // runs per wave
-  static_assert(kPosBytes % 16 == 0, "a position is whole 16-byte words");
-  static_assert(kTB >= 1 && 64 % kTB == 0, "batches must tile 64 tilts");
-  static_assert(kNSlot % (kTX * kTZ / 64) == 0, "runs split evenly on waves");
-};
-
-// Window of one tilt over the tile's active voxels [x0, x1] x [z0, z1]:
-// smallest i0 and smallest k over the four corners.
-template <bool MULTI>
-__device__ __forceinline__ void tile_window(float4 cf, float x0, float x1,
-                                            float z0, float z1, float dscale,
-                                            float doffset, int ndf, int &wb,
-                                            int &kb) {
-  int w = 0x7fffffff, k = 0x7fffffff;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const float x = (c & 1) ? x1 : x0;
-    const float z = (c & 2) ? z1 : z0;
-    float tt;
-    w = min(w, tap_index(x, z, cf, tt));
-    if constexpr (MULTI) k = min(k, slab_of(x, z, cf, dscale, doffset, ndf));
-  }
-  wb = w;
-  kb = MULTI ? k : 0;
-}
-
-__device__ __forceinline__ float readlane_f(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 template <int YB, int STORE, bool MULTI, int KS>
 __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
     const typename StoreT<STORE>::T *__restrict__ slab,
-    const float4 *__restrict__ coef, const uint32_t *__restrict__ tmax,
-    float nrm_mul, float dscale, float doffset, int ndf, int N, int T,
-    int NYB, int W, int Z, int Y, int yb0, int y_base,
-    float *__restrict__ vol, int nbx, int nbz, unsigned nblocks) {
+    const float4 *__restrict__ coef, const int2 *__restrict__ win,
+    float dscale, float doffset, int ndf, int N, int T, int NYB, int W, int Z,
+    int Y, int yb0, int y_base, float *__restrict__ vol, int nbx, int nbz,
+    unsigned nblocks) {
   using S = Staging<YB, STORE, KS>;
   using ST = typename StoreT<STORE>::T;
-  constexpr bool kReduced = STORE != kF32;
+  constexpr bool kExact = STORE == kF32;
   static_assert(MULTI || KS == 1, "one slab without defocus slabs");
-  __shared__ uint4 lds[2 * S::kBufBytes / 16];
+  __shared__ uint4 lds[2 * S::kBufWords];
 
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
   const int bz = (int)(lb % (unsigned)nbz);
@@ -289,33 +389,12 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   const bool active = ix < W && iz < Z;
   const float x = voxel_coord(ix, W);
   const float z = voxel_coord(iz, Z);
-  const float tx0 = voxel_coord(bx * kTX, W);
-  const float tx1 = voxel_coord(min(bx * kTX + kTX, W) - 1, W);
-  const float tz0 = voxel_coord(bz * kTZ, Z);
-  const float tz1 = voxel_coord(min(bz * kTZ + kTZ, Z) - 1, Z);
+  const int2 *tw = win + (size_t)(bx * nbz + bz) * bp_tilt_pad(T);
 
   const size_t row_len = (size_t)N * YB;
   const size_t tstride = (size_t)NYB * row_len;
   const size_t kstride = (size_t)T * tstride;
   const ST *yblock = slab + (size_t)byb * row_len;
-
-  // Per-lane tables of a 64-tilt chunk (lane j: tilt tc + j): coefficients
-  // (the reduced forms carry their per-tilt scale in .w), window base and
-  // first staged slab. pf: chunk of the batch being loaded; cp: chunk of the
-  // batch being interpolated.
-  float4 cpf;
-  int wpf, kpf;
-  auto chunk_tables = [&](int tc) {
-    const int t = tc + lane;
-    cpf = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    wpf = kpf = 0;
-    if (t < T) {
-      cpf = coef[t];
-      tile_window<MULTI>(cpf, tx0, tx1, tz0, tz1, dscale, doffset, ndf, wpf,
-                         kpf);
-      if constexpr (kReduced) cpf.w = reduced_scale(cpf.w, tmax, t, nrm_mul);
-    }
-  };
 
   uint4 pre[S::kSPW][S::kCPL];
   auto load_batch = [&](int tb) {
@@ -324,18 +403,17 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
       const int q = wave + (kTX * kTZ / 64) * i;
       const int j = q / KS, sl = q % KS;
       const int t = tb + j;
-      const int tl = t & 63;
-      const int wb = __builtin_amdgcn_readlane(wpf, tl);
-      const int k = MULTI ? __builtin_amdgcn_readlane(kpf, tl) + sl : 0;
+      const int2 w = kload(tw, t);
+      const int k = MULTI ? w.y + sl : 0;
       const bool ok = t < T && k < ndf;
       const ST *src = yblock + (size_t)t * tstride + (size_t)k * kstride;
 #pragma unroll
       for (int c = 0; c < S::kCPL; ++c) {
         const int ch = lane + 64 * c;
-        const int pos = wb + ch / (S::kPosBytes / 16);
+        const int pos = w.x + ch / S::kPosWords;
         uint4 v = make_uint4(0u, 0u, 0u, 0u);
         if (ok && ch < S::kChunks && (unsigned)pos < (unsigned)N)
-          v = *(const uint4 *)(src + ((long)wb * YB +
+          v = *(const uint4 *)(src + ((long)w.x * YB +
                                       (long)ch * (16 / (int)sizeof(ST))));
         pre[i][c] = v;
       }
@@ -349,7 +427,9 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
       for (int c = 0; c < S::kCPL; ++c) {
         const int ch = lane + 64 * c;
         if (ch < S::kChunks)
-          lds[(buf * S::kBufBytes + q * S::kSlotBytes) / 16 + ch] = pre[i][c];
+          lds[buf * S::kBufWords + q * S::kSlotWords +
+              (ch / S::kPosWords) * S::kPosStride + ch % S::kPosWords] =
+              pre[i][c];
       }
     }
   };
@@ -358,60 +438,35 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
 #pragma unroll
   for (int r = 0; r < YB; ++r) acc[r] = 0.0f;
 
-  chunk_tables(0);
   load_batch(0);
   store_batch(0);
   lds_barrier();
-  float4 ccp = cpf;
-  int wcp = wpf, kcp = kpf;
   const int nb = (T + S::kTB - 1) / S::kTB;
   for (int b = 0; b < nb; ++b) {
     const int tb = b * S::kTB;
-    if ((tb & 63) == 0) {
-      ccp = cpf;
-      wcp = wpf;
-      kcp = kpf;
-    }
     const bool more = b + 1 < nb;
-    if (more) {
-      if (((tb + S::kTB) & 63) == 0) chunk_tables(tb + S::kTB);
-      load_batch(tb + S::kTB);
-    }
-    const int boff = (b & 1) * (S::kBufBytes / 16);
+    if (more) load_batch(tb + S::kTB);
+    const int boff = (b & 1) * S::kBufWords;
+#pragma unroll
     for (int j = 0; j < S::kTB; ++j) {
       const int t = tb + j;
-      if (t >= T) break;
-      const int tl = t & 63;
-      const float4 cf =
-          make_float4(readlane_f(ccp.x, tl), readlane_f(ccp.y, tl),
-                      readlane_f(ccp.z, tl), readlane_f(ccp.w, tl));
-      int ks = 0;
-      if constexpr (MULTI) {
-        ks = slab_of(x, z, cf, dscale, doffset, ndf) -
-             __builtin_amdgcn_readlane(kcp, tl);
-        ks = min(max(ks, 0), KS - 1);
-      }
-      float tt;
-      const int i0 = tap_index(x, z, cf, tt);
-      if (active && (unsigned)i0 < (unsigned)(N - 1)) {
-        int rel = i0 - __builtin_amdgcn_readlane(wcp, tl);
-        rel = min(max(rel, 0), kWPOS - 2);
-        Taps<YB, STORE> tp;
-        tp.load16(&lds[boff + ((j * KS + ks) * S::kSlotBytes +
-                               rel * S::kPosBytes) / 16]);
-        if constexpr (!kReduced) {
-          const float w0 = 1.0f - tt;
-#pragma unroll
-          for (int r = 0; r < YB; ++r)
-            acc[r] = acc[r] + cf.w * (w0 * tp.v0[r] + tt * tp.v1[r]);
-        } else {
-          const float w1 = cf.w * tt;  // cf.w: scale x normalisation
-          const float w0 = cf.w - w1;
-#pragma unroll
-          for (int r = 0; r < YB; ++r) {
-            acc[r] = fmaf(w0, tp.v0[r], acc[r]);
-            acc[r] = fmaf(w1, tp.v1[r], acc[r]);
-          }
+      if (t < T) {
+        const float4 cf = kload(coef, t);
+        const int2 w = kload(tw, t);
+        int ks = 0;
+        if constexpr (MULTI) {
+          ks = slab_of(x, z, cf, dscale, doffset, ndf) - w.y;
+          ks = min(max(ks, 0), KS - 1);
+        }
+        float tt;
+        const int i0 = tap_index<kExact>(x, z, cf, tt);
+        if (active && (unsigned)i0 < (unsigned)(N - 1)) {
+          const int rel = min(max(i0 - w.x, 0), kWPOS - 2);
+          const uint4 *q0 = &lds[boff + (j * KS + ks) * S::kSlotWords +
+                                 rel * S::kPosStride];
+          Taps<YB, STORE> tp;
+          tp.load16(q0, q0 + S::kPosStride);
+          accumulate<YB, STORE>(acc, tp, cf, tt);
         }
       }
     }
@@ -427,18 +482,21 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   }
 }
 
+// ------------------------------------------------------------ launchers ---
+
 template <int YB, int STORE, bool MULTI, int KS>
 static hipError_t launch_bp_lds_k(const BpArgs &a, hipStream_t s) {
   using ST = typename StoreT<STORE>::T;
   const int nbx = (a.W + kTX - 1) / kTX;
   const int nbz = (a.Z + kTZ - 1) / kTZ;
   if (a.yb1 <= a.yb0) return hipSuccess;
+  if (a.win == nullptr) return hipErrorInvalidValue;
   const unsigned nb =
       (unsigned)nbx * (unsigned)nbz * (unsigned)(a.yb1 - a.yb0);
   hipLaunchKernelGGL((k_bp_lds<YB, STORE, MULTI, KS>), dim3(nb),
-                     dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef, a.tmax,
-                     a.nrm_mul, a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB,
-                     a.W, a.Z, a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
+                     dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef, a.win,
+                     a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
+                     a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
   return hipGetLastError();
 }
 
@@ -462,14 +520,14 @@ static hipError_t launch_bp_t(const BpArgs &a, hipStream_t s) {
       (unsigned)nbx * (unsigned)nbz * (unsigned)(a.yb1 - a.yb0);
   if (a.ndf > 1) {
     hipLaunchKernelGGL((k_bp<YB, STORE, true>), dim3(nb), dim3(kTX * kTZ), 0,
-                       s, (const ST *)a.slab, a.coef, a.tmax, a.nrm_mul,
-                       a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
-                       a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
+                       s, (const ST *)a.slab, a.coef, a.dscale, a.doffset,
+                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base,
+                       a.vol, nbx, nbz, nb);
   } else {
     hipLaunchKernelGGL((k_bp<YB, STORE, false>), dim3(nb), dim3(kTX * kTZ), 0,
-                       s, (const ST *)a.slab, a.coef, a.tmax, a.nrm_mul,
-                       a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
-                       a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
+                       s, (const ST *)a.slab, a.coef, a.dscale, a.doffset,
+                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base,
+                       a.vol, nbx, nbz, nb);
   }
   return hipGetLastError();
 }

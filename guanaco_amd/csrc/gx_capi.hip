@@ -111,14 +111,14 @@ void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
   *doffset = -ds * (mn / ps);
 }
 
-// Slabs the LDS-staged backprojector must hold per tilt: a 64 x 4 voxel tile
-// spans at most 63.08 units of h (its diagonal), i.e. 63.08 |dscale| slab
-// widths, so at most ceil of that plus one consecutive slabs (63.2 leaves a
+// Slabs the LDS-staged backprojector must hold per tilt: a 32 x 8 voxel tile
+// spans at most 31.79 units of h (31 |s| + 7 |c|), i.e. 31.79 |dscale| slab
+// widths, so at most ceil of that plus one consecutive slabs (31.9 leaves a
 // margin for rounding). Rounded up to a compiled form (1, 2 or 4); 0 when
 // more are needed and the direct kernel must run.
 int bp_slots(size_t ndf, float dscale) {
   if (ndf <= 1) return 1;
-  const double span = 63.2 * std::fabs((double)dscale);
+  const double span = 31.9 * std::fabs((double)dscale);
   size_t need = ndf;
   if (std::isfinite(span) && span < (double)ndf)
     need = std::min(ndf, (size_t)std::ceil(span) + 1);
@@ -418,8 +418,7 @@ static int bp_slice(const gx_recon_cfg *c, const float *d_slab, float *d_rec,
   BpArgs a{};
   a.slab = d_slab;
   a.coef = d_coef.as<float4>();
-  a.tmax = nullptr;
-  a.nrm_mul = 1.0f;
+  a.win = nullptr;
   defocus_coefficients(c->num_defocus, c->pixel_size, c->min_defocus,
                        c->max_defocus, &a.dscale, &a.doffset);
   a.ndf = (int)c->num_defocus;
@@ -593,7 +592,10 @@ struct gx_volume_plan {
   bool ctf = false;
   float dscale = 0, doffset = 0, nrm_mul = 1;
   size_t slab_bytes = 0, total_bytes = 0;
-  DevBuf coef, wscale, tmax, ramp, mask, defocus, F, G, slab;
+  // coef: the reference's per-tilt constants; coef_r: the reduced forms'
+  // table (bp_coefficients), rebuilt by every backprojection; win: the staged
+  // kernel's per-(tile, tilt) windows. Tilt tables hold bp_tilt_pad(T) rows.
+  DevBuf coef, coef_r, win, wscale, tmax, ramp, mask, defocus, F, G, slab;
 };
 
 namespace {
@@ -630,7 +632,7 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
   } else if (filter_length(X) > 8192) {
     return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", X);
   }
-  ps->YB = d->storage == GX_STORE_F32 ? 8 : 16;
+  ps->YB = d->storage == GX_STORE_F32 ? 16 : 32;
   ps->NYB = (int)((Y + ps->YB - 1) / ps->YB);
   ps->XB = (int)(X / 2 / CB + 1);
   ps->L = (int)(ps->ctf ? 2 * X : filter_length(X));
@@ -654,7 +656,10 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
     ps->f_bytes = ps->g_bytes = 0;
   }
   ps->chunk = (int)chunk;
-  ps->small_bytes = T * (sizeof(float4) + 2 * sizeof(float)) +
+  ps->small_bytes = (size_t)bp_tilt_pad((int)T) * 2 * sizeof(float4) +
+                    T * 2 * sizeof(float) +
+                    bp_window_entries((int)d->width, (int)d->depth, (int)T) *
+                        sizeof(int2) +
                     sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf;
   ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
               ps->small_bytes;
@@ -712,19 +717,35 @@ extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
   p->bp_kind = p->bp_staged ? GX_BP_STAGED : GX_BP_DIRECT;
 
   const size_t T = d->num_angles;
-  std::vector<float> coef(4 * T), w(T, 1.0f);
+  const size_t Tpad = (size_t)bp_tilt_pad((int)T);
+  std::vector<float> coef(4 * Tpad, 0.0f), w(T, 1.0f);
   tilt_geometry(T, d->angles, d->centre, coef.data());
   if (d->weights)
     for (size_t t = 0; t < T; ++t) w[t] = d->weights[t];
   std::vector<float> ramp = ramp_table(d->num_pixels, (size_t)ps.L);
 
-  GX_HIP(p->coef.alloc(sizeof(float4) * T));
+  GX_HIP(p->coef.alloc(sizeof(float4) * Tpad));
+  GX_HIP(p->coef_r.alloc(sizeof(float4) * Tpad));
+  const size_t nwin = bp_window_entries(p->W, p->Z, p->T);
+  GX_HIP(p->win.alloc(sizeof(int2) * nwin));
   GX_HIP(p->wscale.alloc(sizeof(float) * T));
   GX_HIP(p->tmax.alloc(sizeof(uint32_t) * T));
   GX_HIP(p->ramp.alloc(sizeof(float) * ramp.size()));
   GX_HIP(p->slab.alloc(ps.slab_bytes));
-  GX_HIP(hipMemcpy(p->coef.p, coef.data(), sizeof(float4) * T,
+  GX_HIP(hipMemcpy(p->coef.p, coef.data(), sizeof(float4) * Tpad,
                    hipMemcpyHostToDevice));
+  // the staged kernel's windows, with the arithmetic of the plan's kernels
+  if (p->storage == GX_STORE_F32) {
+    GX_HIP(launch_bp_windows(p->coef.as<float4>(), 1, p->T, p->W, p->Z,
+                             p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
+                             nullptr));
+  } else {
+    GX_HIP(launch_bp_coefficients(p->coef.as<float4>(), nullptr, 1.0f, p->T,
+                                  p->coef_r.as<float4>(), nullptr));
+    GX_HIP(launch_bp_windows(p->coef_r.as<float4>(), 0, p->T, p->W, p->Z,
+                             p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
+                             nullptr));
+  }
   GX_HIP(hipMemcpy(p->wscale.p, w.data(), sizeof(float) * T,
                    hipMemcpyHostToDevice));
   GX_HIP(hipMemcpy(p->ramp.p, ramp.data(), sizeof(float) * ramp.size(),
@@ -803,11 +824,17 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   GX_CHECK(y0 <= y1 && y1 <= (size_t)p->Y);
   GX_CHECK(y0 % (size_t)p->YB == 0);
   GX_CHECK(y1 % (size_t)p->YB == 0 || y1 == (size_t)p->Y);
+  hipStream_t s = (hipStream_t)stream;
   BpArgs a{};
   a.slab = p->slab.p;
-  a.coef = p->coef.as<float4>();
-  a.tmax = p->storage != GX_STORE_F32 ? p->tmax.as<uint32_t>() : nullptr;
-  a.nrm_mul = p->nrm_mul;
+  if (p->storage == GX_STORE_F32) {
+    a.coef = p->coef.as<float4>();
+  } else {
+    GX_HIP(launch_bp_coefficients(p->coef.as<float4>(), p->tmax.as<uint32_t>(),
+                                  p->nrm_mul, p->T, p->coef_r.as<float4>(), s));
+    a.coef = p->coef_r.as<float4>();
+  }
+  a.win = p->win.as<int2>();
   a.dscale = p->dscale;
   a.doffset = p->doffset;
   a.ndf = p->ndf;
@@ -822,7 +849,7 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   a.y_base = (int)y0;
   a.vol = d_vol;
   a.staged = p->bp_kind == GX_BP_STAGED ? p->bp_staged : 0;
-  GX_HIP(launch_bp(a, p->YB, p->storage, (hipStream_t)stream));
+  GX_HIP(launch_bp(a, p->YB, p->storage, s));
   return GX_OK;
 }
 

@@ -8,19 +8,30 @@
 
 namespace gx {
 
+// Tables of per-tilt values are padded with zero entries to a multiple of
+// kBpTiltPad, so that kernels can read whole batches of tilts.
+constexpr int kBpTiltPad = 8;
+__host__ __device__ constexpr int bp_tilt_pad(int T) {
+  return (T + kBpTiltPad - 1) / kBpTiltPad * kBpTiltPad;
+}
+
 struct BpArgs {
   const void *slab;          // [k][t][y/YB][x][y%YB]
-  const float4 *coef;        // per tilt: c, s, offset, scale (reference math)
-  const uint32_t *tmax;      // per tilt normalisation bits (reduced storage)
-  float nrm_mul;             // normalisation = float(tmax[t]) * nrm_mul
+  // per tilt (bp_tilt_pad(T) entries):
+  //   kF32:    c, s, offset, scale (the reference's arithmetic)
+  //   reduced: c, s, offset - 1/2, scale * normalisation (bp_coefficients)
+  const float4 *coef;
+  const int2 *win;           // staged kernel: per (tile, tilt) window base and
+                             // first slab (bp_windows)
   float dscale, doffset;     // slab selection (reconstructor.cpp:29-32)
   int ndf, N, T, NYB, W, Z, Y;
   int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
   int y_base;                // vol holds rows [y_base, ...)
   float *vol;                // [y - y_base][z][x]
-  int staged;                // 0: direct kernel; 1, 2, 4: LDS-staged kernel
-                             // holding that many slabs per tilt
+  int staged;                // 0: direct kernel; 2 or 4 (or 1 with one slab):
+                             // LDS-staged kernel holding that many slabs
 };
+
 
 // Each returns hipSuccess or the launch error.
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
@@ -62,6 +73,17 @@ hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
 
 // store: kF32 (exact reference arithmetic) or kF16/kBF16 (FP32 accumulate).
 hipError_t launch_bp(const BpArgs &a, int YB, int store, hipStream_t s);
+// Reduced-storage coefficient table from the reference's (c, s, offset,
+// scale) and the per-tilt normalisation of the filter pass.
+hipError_t launch_bp_coefficients(const float4 *coef, const uint32_t *tmax,
+                                  float nrm_mul, int T, float4 *out,
+                                  hipStream_t s);
+// Window table of the staged kernel: bp_window_entries(W, Z, T) int2, from
+// the coefficient table the kernels read (exact != 0: kF32 arithmetic).
+size_t bp_window_entries(int W, int Z, int T);
+hipError_t launch_bp_windows(const float4 *coef, int exact, int T, int W,
+                             int Z, float dscale, float doffset, int ndf,
+                             int2 *win, hipStream_t s);
 
 // Seeded synthetic tilt series (ellipsoid phantom, optional CTF, noise).
 struct SynthArgs {

@@ -2,13 +2,16 @@
 the kernel's float32 arithmetic (numpy float32 operations are correctly
 rounded, as the kernel's are; it is compiled without FMA contraction).
 
-For every 64 x 4 voxel tile and tilt:
+For every 32 x 8 voxel tile and tilt:
 - the smallest tap index i0 over the tile is the smallest over its four
   corners (each operation producing p is monotone in x and in z);
-- every i0 lies within kWPOS - 2 = 70 positions of that corner minimum, so
-  both taps are inside the kWPOS = 72 staged positions;
+- every i0 lies within kWPOS - 2 = 38 positions of that corner minimum, so
+  both taps are inside the kWPOS = 40 staged positions;
 - the slab indices k of the tile's voxels lie within the slot count the host
-  picks, min(ndf, ceil(63.2 |dscale|) + 1).
+  picks, min(ndf, ceil(31.9 |dscale|) + 1).
+Both the exact (FP32 storage) and the fused multiply-add (FP16/BF16
+storage) forms of p are checked; a float32 fused multiply-add is emulated in
+float64 (the product is exact there, the sum is rounded twice).
 Coefficients come from the oracle's restatement of reconstructor.cpp:36-47.
 """
 import math
@@ -18,7 +21,7 @@ import pytest
 
 from oracle import oracle as O
 
-KTX, KTZ, KWPOS = 64, 4, 72
+KTX, KTZ, KWPOS = 32, 8, 40
 f32 = numpy.float32
 
 
@@ -27,10 +30,18 @@ def voxel_coord(i, n):
     return (numpy.asarray(i, numpy.float64) - 0.5 * n + 0.5).astype(f32)
 
 
-def tap_index(x, z, cf):
+def fma32(a, b, c):
+    return (numpy.asarray(a, numpy.float64) * numpy.asarray(b, numpy.float64)
+            + numpy.asarray(c, numpy.float64)).astype(f32)
+
+
+def tap_index(x, z, cf, exact=True):
     c, s, off = f32(cf[0]), f32(cf[1]), f32(cf[2])
-    p = (x * c - z * s) + off
-    p = p - f32(0.5)
+    if exact:
+        p = (x * c - z * s) + off
+        p = p - f32(0.5)
+    else:  # the reduced forms' table holds offset - 1/2
+        p = fma32(x, c, fma32(-z, s, off - f32(0.5)))
     return numpy.floor(p).astype(numpy.int64)
 
 
@@ -47,7 +58,7 @@ def host_slots(ndf, dscale):
     # gx_capi.hip bp_slots before rounding up to a compiled form
     if ndf <= 1:
         return 1
-    span = 63.2 * abs(float(dscale))
+    span = 31.9 * abs(float(dscale))
     if not math.isfinite(span) or span >= ndf:
         return ndf
     return min(ndf, math.ceil(span) + 1)
@@ -88,8 +99,9 @@ GEOMETRIES = [
 ]
 
 
+@pytest.mark.parametrize("exact", [True, False])
 @pytest.mark.parametrize("geom", GEOMETRIES)
-def test_staged_window_holds_every_tap(geom):
+def test_staged_window_holds_every_tap(geom, exact):
     N, W, Z, centre, (a0, a1), T = geom
     rng = numpy.random.default_rng(N + T)
     angles = numpy.deg2rad(numpy.linspace(a0, a1, T)) + rng.uniform(
@@ -100,13 +112,14 @@ def test_staged_window_holds_every_tap(geom):
     big = numpy.iinfo(numpy.int64).max
     worst = 0
     for t in range(T):
-        i0 = tap_index(x, z, coef[t])
-        wb = corner_min(lambda xc, zc: tap_index(xc, zc, coef[t]), W, Z)
+        i0 = tap_index(x, z, coef[t], exact)
+        wb = corner_min(lambda xc, zc: tap_index(xc, zc, coef[t], exact),
+                        W, Z)
         lo = tiles(i0, W, Z, big).min(axis=2)
         hi = tiles(i0, W, Z, -big).max(axis=2)
         numpy.testing.assert_array_equal(lo, wb)
         worst = max(worst, int((hi - wb).max()))
-    assert worst <= 64, worst
+    assert worst <= 32, worst
     assert worst <= KWPOS - 2
 
 

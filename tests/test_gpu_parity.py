@@ -311,7 +311,7 @@ BP_KIND_CASES = [
     # every quadrant (negative cosines), T > 64: several tilt chunks
     (64, 16, 150, 1, 13, 64, (-180.0, 178.0), (0.0, 0.0), 1.0),
     # three slabs per tile at most -> four staged
-    (128, 24, 130, 5, 40, 128, (-70.0, 70.0), (-300.0, 300.0), 2.0),
+    (128, 24, 130, 16, 40, 128, (-70.0, 70.0), (-128.0, 128.0), 1.0),
     # two slabs per tile at most
     (64, 16, 21, 8, 64, 200, (-60.0, 60.0), (-512.0, 512.0), 1.0),
     # narrow slabs: more than four per tile, direct kernel only
