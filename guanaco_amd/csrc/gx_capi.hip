@@ -180,15 +180,22 @@ const float2 *twiddles(hipError_t *err) {
   }
   std::lock_guard<std::mutex> lock(mu);
   if (tables[dev] == nullptr) {
-    std::vector<float2> h(kTwN);
+    // W^j, then the per-thread blocks of every transform shape (gx_fft.hpp)
+    std::vector<float2> h(tw_table_len());
     for (int j = 0; j < kTwN; ++j) {
       const double a = 2.0 * M_PI * (double)j / (double)kTwN;
       h[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
     }
+    for (int ln = 1; ln <= 14; ++ln)
+      for (int e = 2; e <= 16; e *= 2)
+        if (tw_count(1 << ln, e) > 0)
+          tw_fill_block(1 << ln, e, h.data(),
+                        h.data() + kTwN + tw_block_offset(1 << ln, e));
     float2 *d = nullptr;
-    *err = hipMalloc(&d, sizeof(float2) * kTwN);
+    *err = hipMalloc(&d, sizeof(float2) * h.size());
     if (*err != hipSuccess) return nullptr;
-    *err = hipMemcpy(d, h.data(), sizeof(float2) * kTwN, hipMemcpyHostToDevice);
+    *err = hipMemcpy(d, h.data(), sizeof(float2) * h.size(),
+                     hipMemcpyHostToDevice);
     if (*err != hipSuccess) {
       (void)hipFree(d);
       return nullptr;

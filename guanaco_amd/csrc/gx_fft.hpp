@@ -228,52 +228,74 @@ __device__ __forceinline__ void fft_sync() {
 }
 
 // Per-thread twiddle factors of one N-point transform with E values per
-// thread (raw table values W^j = exp(-2 pi i j / kTwN); a transform with
-// DIR = +1 conjugates them). Loaded once, before the first stage, so that
-// their latency overlaps the first radix pass, and reusable by every
-// transform of that size the thread computes.
+// thread (raw values W^j = exp(-2 pi i j / kTwN); a transform with DIR = +1
+// conjugates them). Loaded once, before the first stage, so that their
+// latency overlaps the first radix pass, and reusable by every transform of
+// that size the thread computes.
 //
 // Middle stage S multiplies element m of each group by W^(m k kTwN/(Ns R)),
 // k = b mod Ns. Every b = tid + q N/E of a thread has the same k (Ns divides
 // N/E), so a stage needs R - 1 values. The last stage needs W^(m tid kTwN/N)
 // for m < E: formed as lo[m mod 4] * hi[m / 4] (one rounding) from at most
 // three values of each kind.
+//
+// The device table holds W^j (kTwN entries) followed, for every (N, E), by
+// these per-thread values laid out [value][tid], so that each load of a
+// wave is one contiguous run.
+__host__ __device__ constexpr int tw_mid_before(int N, int E, int S) {
+  int c = 0;
+  const int ns = plan_stages(N, E);
+  for (int s = 1; s < S && s < ns - 1; ++s) c += plan_radix(N, E, s) - 1;
+  return c;
+}
+__host__ __device__ constexpr int tw_lo_count(int E) { return E - 1 < 3 ? E - 1 : 3; }
+__host__ __device__ constexpr int tw_hi_count(int E) { return E / 4 > 1 ? E / 4 - 1 : 0; }
+// per-thread values of (N, E); 0 for single-stage or invalid shapes
+__host__ __device__ constexpr int tw_count(int N, int E) {
+  if (N <= E || N < E * E) return 0;
+  return tw_mid_before(N, E, plan_stages(N, E)) + tw_lo_count(E) +
+         tw_hi_count(E);
+}
+// offset of the (N, E) block after the kTwN-entry table; blocks are ordered
+// by log2 N (1..14), then E = 2, 4, 8, 16
+__host__ __device__ constexpr size_t tw_block_offset(int N, int E) {
+  size_t off = 0;
+  for (int ln = 1; ln <= 14; ++ln) {
+    for (int e = 2; e <= 16; e *= 2) {
+      if ((1 << ln) == N && e == E) return off;
+      off += (size_t)tw_count(1 << ln, e) * (size_t)((1 << ln) / e);
+    }
+  }
+  return off;
+}
+__host__ __device__ constexpr size_t tw_table_len() {
+  return (size_t)kTwN + tw_block_offset(1 << 15, 2);
+}
+
 template <int N, int E>
 struct FftTwiddles {
   static constexpr int NS = plan_stages(N, E);
-  static constexpr int mid_before(int S) {
-    int c = 0;
-    for (int s = 1; s < S && s < NS - 1; ++s) c += plan_radix(N, E, s) - 1;
-    return c;
-  }
-  static constexpr int kMid = mid_before(NS);
-  static constexpr int kLo = E - 1 < 3 ? E - 1 : 3;      // W^(m1 tid), m1 = 1..3
-  static constexpr int kHi = E / 4 > 1 ? E / 4 - 1 : 0;  // W^(4 m2 tid)
+  static constexpr int NT = N / E;
+  static constexpr int kMid = NS > 1 ? tw_mid_before(N, E, NS) : 0;
+  static constexpr int kLo = tw_lo_count(E);
+  static constexpr int kHi = tw_hi_count(E);
+  static constexpr int mid_before(int S) { return tw_mid_before(N, E, S); }
+  static constexpr size_t kOffset = kTwN + tw_block_offset(N, E);
   float2 mid[kMid > 0 ? kMid : 1];
   float2 lo[kLo > 0 ? kLo : 1];
   float2 hi[kHi > 0 ? kHi : 1];
 
-  template <int S>
-  __device__ __forceinline__ void load_mid(int tid,
-                                           const float2 *__restrict__ tw) {
-    if constexpr (S < NS - 1) {
-      constexpr int R = plan_radix(N, E, S);
-      constexpr int Ns = plan_ns(N, E, S);
-      constexpr int TWS = kTwN / (Ns * R);
-      const int k = tid & (Ns - 1);
-#pragma unroll
-      for (int m = 1; m < R; ++m) mid[mid_before(S) + m - 1] = tw[m * k * TWS];
-      load_mid<S + 1>(tid, tw);
-    }
-  }
+  // tw: the device table (twiddles())
   __device__ __forceinline__ void load(int tid, const float2 *__restrict__ tw) {
     if constexpr (NS > 1) {
-      load_mid<1>(tid, tw);
-      constexpr int TWS = kTwN / N;
+      static_assert(tw_count(N, E) == kMid + kLo + kHi, "table layout");
+      const float2 *t = tw + kOffset + tid;
 #pragma unroll
-      for (int m = 1; m <= kLo; ++m) lo[m - 1] = tw[m * tid * TWS];
+      for (int i = 0; i < kMid; ++i) mid[i] = t[i * NT];
 #pragma unroll
-      for (int m = 1; m <= kHi; ++m) hi[m - 1] = tw[4 * m * tid * TWS];
+      for (int i = 0; i < kLo; ++i) lo[i] = t[(kMid + i) * NT];
+#pragma unroll
+      for (int i = 0; i < kHi; ++i) hi[i] = t[(kMid + kLo + i) * NT];
     }
   }
   // last-stage factor for element m (0 < m < E)
@@ -285,6 +307,26 @@ struct FftTwiddles {
     return cmul(lo[m1 - 1], hi[m2 - 1]);
   }
 };
+
+// Host: the per-thread block of (N, E) from the W^j table w (kTwN entries),
+// the same values FftTwiddles would take from it.
+inline void tw_fill_block(int N, int E, const float2 *w, float2 *out) {
+  const int NT = N / E, ns = plan_stages(N, E);
+  const int nmid = tw_mid_before(N, E, ns), nlo = tw_lo_count(E);
+  for (int tid = 0; tid < NT; ++tid) {
+    for (int s = 1; s < ns - 1; ++s) {
+      const int R = plan_radix(N, E, s), Ns = plan_ns(N, E, s);
+      const int TWS = kTwN / (Ns * R), k = tid & (Ns - 1);
+      for (int m = 1; m < R; ++m)
+        out[(tw_mid_before(N, E, s) + m - 1) * NT + tid] = w[m * k * TWS];
+    }
+    const int TWS = kTwN / N;
+    for (int m = 1; m <= nlo; ++m)
+      out[(nmid + m - 1) * NT + tid] = w[m * tid * TWS];
+    for (int m = 1; m <= tw_hi_count(E); ++m)
+      out[(nmid + nlo + m - 1) * NT + tid] = w[4 * m * tid * TWS];
+  }
+}
 
 template <int DIR>
 __device__ __forceinline__ float2 dir_conj(float2 w) {
