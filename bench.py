@@ -244,6 +244,131 @@ def cpu_baseline(w, proj, angles, budget_s):
     }
 
 
+# ------------------------------------------------------------- serving ----
+
+SERVE_SIZES = (512, 1024, 2048)
+SERVE_DESC = ("Poisson arrivals of whole-volume requests: detector S^2 with S "
+              "uniform in {512, 1024, 2048}, T uniform in 41..121 consecutive "
+              "1-degree tilts centred on 0 (slices of one seeded 121-tilt "
+              "+-60 degree series per size, device-resident), ndf uniform in "
+              "1..16, df uniform in 10000..40000 A, Cs 2.7 mm, 300 kV, "
+              "1.5 A/px, volume S x S/4 x S, FP16 slabs; arrival rate 0.7 x "
+              "the capacity measured closed-loop first")
+
+
+def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
+                  log=sys.stderr):
+    """Continuous batching through the native server (gx_server_*):
+    capacity measured closed-loop on 24 requests, then n_requests Poisson
+    arrivals at 0.7 x capacity; latency is submission to completion."""
+    import threading
+    from guanaco_amd.serve import Server
+    from oracle import oracle as O  # only for the wavelength constant
+    l = O.electron_wavelength(300000)
+    ps = 1.5
+    angles121 = numpy.deg2rad(numpy.arange(121) - 60.0).astype(numpy.float32)
+    series = {}
+    for k, S in enumerate(SERVE_SIZES):
+        d = C.DeviceArray((121, S, S))
+        C.synth_projections(d, S, S, angles121, depth=S // 4, width=S,
+                            num_ellipsoids=100, seed=1100 + k,
+                            semi=(S / 64.0, S / 8.0), noise="poisson",
+                            noise_level=40.0,
+                            ctf=dict(wavelength=l, Cs=2.7e7, defocus=25000.0,
+                                     phi=math.asin(0.1), pixel_size=ps),
+                            stream=stream)
+        series[S] = d
+    stream.synchronize()
+
+    def make(rng):
+        S = int(SERVE_SIZES[rng.integers(len(SERVE_SIZES))])
+        T = int(rng.integers(41, 122))
+        ndf = int(rng.integers(1, 17))
+        df = float(rng.uniform(10000.0, 40000.0))
+        start = 60 - (T - 1) // 2
+        view = C.DeviceView(series[S].ptr + start * S * S * 4, (T, S, S))
+        centre = numpy.full(S, S / 2.0, numpy.float32)
+        mn, mx, st, n = _defocus_steps(df, ndf, S, centre, ps)
+        dfs = numpy.array([mn + d * st for d in range(n)], numpy.float64)
+        kw = dict(num_defocus=ndf, pixel_size=ps, min_defocus=mn - df,
+                  max_defocus=mx - df, storage="f16", defocus=dfs,
+                  wavelength=l, Cs=2.7e7, phi=math.asin(0.1))
+        return view, angles121[start:start + T], S // 4, kw
+
+    srv = Server(max_inflight=max_inflight, backfill=False)
+    try:
+        # capacity: 24 requests submitted at once
+        rng = numpy.random.default_rng(seed + 1)
+        reqs = []
+        for _ in range(24):
+            v, a, Z, kw = make(rng)
+            reqs.append(srv.submit(v, a, Z, **kw))
+        infos = []
+        for r in reqs:
+            infos.append(r.wait())
+            r.release()
+        span = max(i["end_s"] for i in infos) - min(i["submit_s"]
+                                                    for i in infos)
+        capacity = len(reqs) / span
+        print("bench: serving capacity %.1f requests/s (%d closed-loop)"
+              % (capacity, len(reqs)), file=log, flush=True)
+        srv.close()
+        srv = Server(max_inflight=max_inflight, backfill=False)
+        rate = 0.7 * capacity
+        rng = numpy.random.default_rng(seed)
+        work = [make(rng) for _ in range(n_requests)]
+        arrivals = numpy.cumsum(rng.exponential(1.0 / rate, n_requests))
+        handles = []
+        done = []
+        lock = threading.Lock()
+        submitted = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(n_requests):
+                submitted.acquire()
+                with lock:
+                    r = handles[i]
+                inf = r.wait()
+                done.append(inf)
+                r.release()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, (v, a, Z, kw) in enumerate(work):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            r = srv.submit(v, a, Z, **kw)
+            with lock:
+                handles.append(r)
+            submitted.release()
+        th.join()
+        st = srv.stats()
+    finally:
+        srv.close()
+        for d in series.values():
+            d.free()
+    lat = [i["latency_ms"] for i in done if i["state"] == "done"]
+    return {
+        "description": SERVE_DESC,
+        "requests": n_requests,
+        "completed": int(st["completed"]),
+        "failed": int(st["failed"]),
+        "capacity_requests_per_s": capacity,
+        "arrival_rate_per_s": rate,
+        "p50_ms": percentile(lat, 50),
+        "p99_ms": percentile(lat, 99),
+        "mean_ms": sum(lat) / max(1, len(lat)),
+        "max_ms": max(lat) if lat else None,
+        "device_ms_mean": sum(i["device_ms"] for i in done) / max(1, len(done)),
+        "updates_per_s": st["updates_per_s"],
+        "max_inflight": max_inflight,
+        "budget_bytes": int(st["budget_bytes"]),
+        "peak_bytes": int(st["peak_bytes"]),
+    }
+
+
 def load_traffic(name):
     """HBM bytes per BP launch from the committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", name)
@@ -264,7 +389,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="c2",
+                    choices=sorted(WORKLOADS) + ["c5"],
+                    help="c5: the serving benchmark only")
+    ap.add_argument("--serve-requests", type=int, default=None,
+                    help="requests of the serving phase (default 64 after "
+                         "the c1-c3 workloads, 256 for c5; 0: skip)")
     ap.add_argument("--storage", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -286,6 +416,8 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.config == "c5":
+        return main_serving(args)
     w = WORKLOADS[args.config]
     check = not args.no_check and args.only is None and w["ndf"] == 1
     # CPU workers for the oracle check are forked before anything touches
@@ -452,12 +584,49 @@ def main():
         print("bench: oracle check %.1f s %s" % (time.perf_counter() - t0,
                                                 line["rel_l2"]),
               file=sys.stderr, flush=True)
+    n_serve = 64 if args.serve_requests is None else args.serve_requests
+    if n_serve > 0 and args.only is None:
+        line["serving"] = serving_bench(C, n_serve, stream)
+        print("bench: serving %s" % {k: v for k, v in line["serving"].items()
+                                     if k != "description"},
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         proj = d_in.to_host()
         line["cpu_baseline"] = cpu_baseline(w, proj, angles, args.cpu_budget)
     for p in plans.values():
         p.destroy()
     print(json.dumps(line), flush=True)
+
+
+def main_serving(args):
+    """--config c5: the serving benchmark as the headline line."""
+    from guanaco_amd import _capi as C
+    if C.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    stream = C.Stream()
+    n = 256 if args.serve_requests is None else args.serve_requests
+    sv = serving_bench(C, n, stream)
+    print(json.dumps({
+        "metric": METRIC,
+        "value": sv["updates_per_s"],
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 0,
+        "ms_per_step": sv["mean_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 100-ellipsoid phantoms, CTF forward model, "
+                "Poisson 40 e-/px",
+        "config": {"workload": SERVE_DESC, "path": "served",
+                   "storage": "f16", "accumulate": "f32",
+                   "requests": n, "max_inflight": sv["max_inflight"]},
+        "latency_ms": {"p50": sv["p50_ms"], "p99": sv["p99_ms"]},
+        "serving": sv,
+    }), flush=True)
 
 
 if __name__ == "__main__":

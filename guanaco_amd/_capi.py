@@ -87,6 +87,29 @@ class SynthDesc(ctypes.Structure):
     ]
 
 
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("budget_bytes", _sz), ("max_inflight", _int),
+                ("backfill", _int), ("gpu_index", _int)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", _int), ("submit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("end_s", ctypes.c_double),
+                ("device_ms", ctypes.c_double), ("updates", ctypes.c_double),
+                ("footprint_bytes", _sz), ("volume", _vp),
+                ("error", ctypes.c_char * 256)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", _sz), ("completed", _sz), ("failed", _sz),
+                ("cancelled", _sz), ("queued", _sz), ("running", _sz),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("updates_per_s", ctypes.c_double),
+                ("span_s", ctypes.c_double), ("budget_bytes", _sz),
+                ("used_bytes", _sz), ("peak_bytes", _sz)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
     "gx_last_error": [],
@@ -136,6 +159,18 @@ SIGNATURES = {
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
     "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],
+    "gx_server_create": [ctypes.POINTER(ServerCfg), ctypes.POINTER(_vp)],
+    "gx_server_destroy": [_vp],
+    "gx_server_request_bytes": [ctypes.POINTER(VolumeDesc), _int,
+                                ctypes.POINTER(_sz)],
+    "gx_server_submit": [_vp, ctypes.POINTER(VolumeDesc), _vp, _int,
+                         ctypes.POINTER(ctypes.c_uint64)],
+    "gx_server_wait": [_vp, ctypes.c_uint64, ctypes.c_double,
+                       ctypes.POINTER(RequestInfo)],
+    "gx_server_info": [_vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)],
+    "gx_server_cancel": [_vp, ctypes.c_uint64],
+    "gx_server_release": [_vp, ctypes.c_uint64],
+    "gx_server_get_stats": [_vp, ctypes.POINTER(ServerStats)],
 }
 _RESTYPE = {"gx_last_error": ctypes.c_char_p}
 
@@ -248,6 +283,24 @@ class DeviceArray:
             pass
 
 
+class DeviceView:
+    """Non-owning view of device memory (e.g. a slice of a DeviceArray)."""
+
+    def __init__(self, ptr, shape, dtype=numpy.float32):
+        self.ptr = int(ptr)
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = numpy.dtype(dtype)
+        self.nbytes = int(numpy.prod(self.shape, dtype=numpy.int64)) * \
+            self.dtype.itemsize
+
+    def to_host(self, stream=None):
+        out = numpy.empty(self.shape, self.dtype)
+        call("gx_memcpy_dtoh", out.ctypes.data, self.ptr, self.nbytes, stream)
+        if stream is not None:
+            call("gx_stream_synchronize", stream)
+        return out
+
+
 class Stream:
     def __init__(self):
         p = _vp()
@@ -290,6 +343,32 @@ class Event:
 # ---- whole-volume plans -------------------------------------------------
 
 
+def volume_desc(num_pixels, num_rows, angles, depth, width=None,
+                num_defocus=1, pixel_size=1.0, centre=None, min_defocus=0.0,
+                max_defocus=0.0, weights=None, storage="f32",
+                input_layout=INPUT_PROJECTIONS, defocus=None, wavelength=0.0,
+                Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0, tilt_chunk=0, gpu_index=-1):
+    """A gx_volume_desc and the arrays it points to (keep them alive)."""
+    angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
+    T = angles.size
+    weights = None if weights is None else numpy.ascontiguousarray(
+        weights, dtype=numpy.float32)
+    defocus = None if defocus is None else numpy.ascontiguousarray(
+        defocus, dtype=numpy.float64)
+    width = num_pixels if width is None else width
+    centre = num_pixels / 2.0 if centre is None else centre
+    desc = VolumeDesc(
+        num_pixels, num_rows, T, num_defocus, depth, width,
+        float(pixel_size), float(centre), float(min_defocus),
+        float(max_defocus), f32p(angles),
+        None if weights is None else f32p(weights),
+        STORAGE[storage] if isinstance(storage, str) else storage,
+        input_layout, None if defocus is None else f64p(defocus),
+        float(wavelength), float(Cs), float(Ca), float(Pa), float(phi),
+        tilt_chunk, gpu_index)
+    return desc, (angles, weights, defocus)
+
+
 class VolumePlan:
     """A device-resident reconstruction plan (gx_volume_plan)."""
 
@@ -299,24 +378,12 @@ class VolumePlan:
                  input_layout=INPUT_PROJECTIONS, defocus=None,
                  wavelength=0.0, Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0,
                  tilt_chunk=0, gpu_index=-1):
-        self.angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
-        T = self.angles.size
-        self.weights = None if weights is None else numpy.ascontiguousarray(
-            weights, dtype=numpy.float32)
-        self.defocus = None if defocus is None else numpy.ascontiguousarray(
-            defocus, dtype=numpy.float64)
+        self.desc, self._keep = volume_desc(
+            num_pixels, num_rows, angles, depth, width, num_defocus,
+            pixel_size, centre, min_defocus, max_defocus, weights, storage,
+            input_layout, defocus, wavelength, Cs, Ca, Pa, phi, tilt_chunk,
+            gpu_index)
         width = num_pixels if width is None else width
-        centre = num_pixels / 2.0 if centre is None else centre
-        self.desc = VolumeDesc(
-            num_pixels, num_rows, T, num_defocus, depth, width,
-            float(pixel_size), float(centre), float(min_defocus),
-            float(max_defocus), f32p(self.angles),
-            None if self.weights is None else f32p(self.weights),
-            STORAGE[storage] if isinstance(storage, str) else storage,
-            input_layout,
-            None if self.defocus is None else f64p(self.defocus),
-            float(wavelength), float(Cs), float(Ca), float(Pa), float(phi),
-            tilt_chunk, gpu_index)
         self.shape = (num_rows, depth, width)
         self.storage = storage
         p = _vp()

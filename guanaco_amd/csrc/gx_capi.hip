@@ -55,13 +55,18 @@ int fail(int code, const char *fmt, ...) {
 // Device buffer released on scope exit.
 struct DevBuf {
   void *p = nullptr;
+  bool owned = true;  // false: a slice of a caller's workspace
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
   }
   hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16); }
+  void borrow(void *q) {
+    p = q;
+    owned = false;
+  }
   template <typename T>
   T *as() const {
     return (T *)p;
@@ -603,9 +608,17 @@ struct gx_volume_plan {
   // table (bp_coefficients), rebuilt by every backprojection; win: the staged
   // kernel's per-(tile, tilt) windows. Tilt tables hold bp_tilt_pad(T) rows.
   DevBuf coef, coef_r, win, wscale, tmax, ramp, mask, defocus, F, G, slab;
+  // host copies of the uploaded tables, alive until the plan is destroyed so
+  // that stream-ordered uploads may read them
+  std::vector<float> h_coef, h_wscale, h_ramp;
+  std::vector<double> h_defocus;
+  double wavelength = 0, Cs = 0, Ca = 0, Pa = 0, phi = 0, pixel_size = 0;
 };
 
 namespace {
+
+constexpr size_t kPlanAlign = 256;
+constexpr int kPlanBuffers = 11;
 
 struct PlanSizes {
   int YB, NYB, XB, L, chunk;
@@ -668,8 +681,9 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
                     bp_window_entries((int)d->width, (int)d->depth, (int)T) *
                         sizeof(int2) +
                     sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf;
+  // every buffer starts on a kPlanAlign boundary inside a workspace
   ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
-              ps->small_bytes;
+              ps->small_bytes + kPlanBuffers * kPlanAlign;
   return GX_OK;
 }
 
@@ -685,22 +699,11 @@ extern "C" int gx_volume_plan_bytes(const gx_volume_desc *desc,
   return GX_OK;
 }
 
-extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
-                                     gx_volume_plan **out) {
-  GX_CHECK(out != nullptr);
-  *out = nullptr;
-  PlanSizes ps;
-  int rc = plan_sizes(d, &ps);
-  if (rc) return rc;
-  rc = select_device(d->gpu_index);
-  if (rc) return rc;
-  hipError_t terr;
-  twiddles(&terr);  // build the table outside any capture
-  GX_HIP(terr);
+namespace {
 
-  auto *p = new gx_volume_plan();
-  std::unique_ptr<gx_volume_plan> guard(p);
-  GX_HIP(hipGetDevice(&p->device));
+// Host fields and host tables of a plan.
+void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
+                const PlanSizes &ps) {
   p->X = (int)d->num_pixels;
   p->Y = (int)d->num_rows;
   p->T = (int)d->num_angles;
@@ -722,60 +725,141 @@ extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
                        d->max_defocus, &p->dscale, &p->doffset);
   p->bp_staged = bp_slots(d->num_defocus, p->dscale);
   p->bp_kind = p->bp_staged ? GX_BP_STAGED : GX_BP_DIRECT;
-
   const size_t T = d->num_angles;
-  const size_t Tpad = (size_t)bp_tilt_pad((int)T);
-  std::vector<float> coef(4 * Tpad, 0.0f), w(T, 1.0f);
-  tilt_geometry(T, d->angles, d->centre, coef.data());
+  p->h_coef.assign(4 * (size_t)bp_tilt_pad((int)T), 0.0f);
+  tilt_geometry(T, d->angles, d->centre, p->h_coef.data());
+  p->h_wscale.assign(T, 1.0f);
   if (d->weights)
-    for (size_t t = 0; t < T; ++t) w[t] = d->weights[t];
-  std::vector<float> ramp = ramp_table(d->num_pixels, (size_t)ps.L);
+    for (size_t t = 0; t < T; ++t) p->h_wscale[t] = d->weights[t];
+  p->h_ramp = ramp_table(d->num_pixels, (size_t)ps.L);
+  if (ps.ctf) p->h_defocus.assign(d->defocus, d->defocus + d->num_defocus);
+  p->wavelength = d->wavelength;
+  p->Cs = d->Cs;
+  p->Ca = d->Ca;
+  p->Pa = d->Pa;
+  p->phi = d->phi;
+  p->pixel_size = d->pixel_size;
+}
 
-  GX_HIP(p->coef.alloc(sizeof(float4) * Tpad));
-  GX_HIP(p->coef_r.alloc(sizeof(float4) * Tpad));
-  const size_t nwin = bp_window_entries(p->W, p->Z, p->T);
-  GX_HIP(p->win.alloc(sizeof(int2) * nwin));
-  GX_HIP(p->wscale.alloc(sizeof(float) * T));
-  GX_HIP(p->tmax.alloc(sizeof(uint32_t) * T));
-  GX_HIP(p->ramp.alloc(sizeof(float) * ramp.size()));
-  GX_HIP(p->slab.alloc(ps.slab_bytes));
-  GX_HIP(hipMemcpy(p->coef.p, coef.data(), sizeof(float4) * Tpad,
-                   hipMemcpyHostToDevice));
+// The plan's device buffers and their sizes, in workspace order.
+void plan_buffers(gx_volume_plan *p, const PlanSizes &ps,
+                  std::vector<std::pair<DevBuf *, size_t>> &out) {
+  const size_t T = p->T, Tpad = (size_t)bp_tilt_pad(p->T);
+  out = {{&p->slab, ps.slab_bytes},
+         {&p->coef, sizeof(float4) * Tpad},
+         {&p->coef_r, sizeof(float4) * Tpad},
+         {&p->win, sizeof(int2) * bp_window_entries(p->W, p->Z, p->T)},
+         {&p->wscale, sizeof(float) * T},
+         {&p->tmax, sizeof(uint32_t) * T},
+         {&p->ramp, sizeof(float) * p->h_ramp.size()},
+         {&p->defocus, ps.ctf ? sizeof(double) * p->ndf : 0},
+         {&p->mask, ps.mask_bytes},
+         {&p->F, ps.f_bytes},
+         {&p->G, ps.g_bytes}};
+}
+
+// Uploads the tables and initialises the device state on stream s.
+int plan_upload(gx_volume_plan *p, hipStream_t s) {
+  const size_t T = p->T, Tpad = (size_t)bp_tilt_pad(p->T);
+  GX_HIP(hipMemcpyAsync(p->coef.p, p->h_coef.data(), sizeof(float4) * Tpad,
+                        hipMemcpyHostToDevice, s));
   // the staged kernel's windows, with the arithmetic of the plan's kernels
   if (p->storage == GX_STORE_F32) {
     GX_HIP(launch_bp_windows(p->coef.as<float4>(), 1, p->T, p->W, p->Z,
                              p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
-                             nullptr));
+                             s));
   } else {
     GX_HIP(launch_bp_coefficients(p->coef.as<float4>(), nullptr, 1.0f, p->T,
-                                  p->coef_r.as<float4>(), nullptr));
+                                  p->coef_r.as<float4>(), s));
     GX_HIP(launch_bp_windows(p->coef_r.as<float4>(), 0, p->T, p->W, p->Z,
                              p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
-                             nullptr));
+                             s));
   }
-  GX_HIP(hipMemcpy(p->wscale.p, w.data(), sizeof(float) * T,
-                   hipMemcpyHostToDevice));
-  GX_HIP(hipMemcpy(p->ramp.p, ramp.data(), sizeof(float) * ramp.size(),
-                   hipMemcpyHostToDevice));
+  GX_HIP(hipMemcpyAsync(p->wscale.p, p->h_wscale.data(), sizeof(float) * T,
+                        hipMemcpyHostToDevice, s));
+  GX_HIP(hipMemcpyAsync(p->ramp.p, p->h_ramp.data(),
+                        sizeof(float) * p->h_ramp.size(),
+                        hipMemcpyHostToDevice, s));
   // rows past Y in the last row block are read by the backprojector and
   // discarded; keep them finite
-  GX_HIP(launch_fill_u32(p->slab.as<uint32_t>(), 0u, ps.slab_bytes / 4,
-                         nullptr));
-  if (ps.ctf) {
-    GX_HIP(p->defocus.alloc(sizeof(double) * d->num_defocus));
-    GX_HIP(hipMemcpy(p->defocus.p, d->defocus,
-                     sizeof(double) * d->num_defocus, hipMemcpyHostToDevice));
-    GX_HIP(p->mask.alloc(ps.mask_bytes));
-    GX_HIP(p->F.alloc(ps.f_bytes));
-    GX_HIP(p->G.alloc(ps.g_bytes));
+  GX_HIP(launch_fill_u32(p->slab.as<uint32_t>(), 0u, p->slab_bytes / 4, s));
+  if (p->ctf) {
+    GX_HIP(hipMemcpyAsync(p->defocus.p, p->h_defocus.data(),
+                          sizeof(double) * p->ndf, hipMemcpyHostToDevice, s));
     GX_HIP(launch_ctf_mask(p->mask.as<int8_t>(), p->X, p->Y, p->XB, p->ndf,
-                           p->defocus.as<double>(), d->wavelength, d->Cs,
-                           d->Ca, d->Pa, d->phi, d->pixel_size, nullptr));
+                           p->defocus.as<double>(), p->wavelength, p->Cs,
+                           p->Ca, p->Pa, p->phi, p->pixel_size, s));
   }
+  return GX_OK;
+}
+
+}  // namespace
+
+extern "C" int gx_volume_plan_create(const gx_volume_desc *d,
+                                     gx_volume_plan **out) {
+  GX_CHECK(out != nullptr);
+  *out = nullptr;
+  PlanSizes ps;
+  int rc = plan_sizes(d, &ps);
+  if (rc) return rc;
+  rc = select_device(d->gpu_index);
+  if (rc) return rc;
+  hipError_t terr;
+  twiddles(&terr);  // build the table outside any capture
+  GX_HIP(terr);
+
+  auto *p = new gx_volume_plan();
+  std::unique_ptr<gx_volume_plan> guard(p);
+  GX_HIP(hipGetDevice(&p->device));
+  plan_setup(p, d, ps);
+  std::vector<std::pair<DevBuf *, size_t>> bufs;
+  plan_buffers(p, ps, bufs);
+  for (auto &b : bufs)
+    if (b.second) GX_HIP(b.first->alloc(b.second));
+  rc = plan_upload(p, nullptr);
+  if (rc) return rc;
   GX_HIP(hipDeviceSynchronize());
   *out = guard.release();
   return GX_OK;
 }
+
+namespace gx {
+
+// A plan whose buffers are slices of `ws` (gx_volume_plan_bytes bytes,
+// kPlanAlign-aligned), initialised on `s` with no allocation and no host
+// synchronisation (for the serving loop, which owns the memory).
+int plan_create_in(const gx_volume_desc *d, void *ws, size_t bytes,
+                   hipStream_t s, gx_volume_plan **out) {
+  GX_CHECK(out != nullptr && ws != nullptr);
+  GX_CHECK(((uintptr_t)ws % kPlanAlign) == 0);
+  *out = nullptr;
+  PlanSizes ps;
+  int rc = plan_sizes(d, &ps);
+  if (rc) return rc;
+  GX_CHECK(bytes >= ps.total);
+  hipError_t terr;
+  twiddles(&terr);
+  GX_HIP(terr);
+  auto *p = new gx_volume_plan();
+  std::unique_ptr<gx_volume_plan> guard(p);
+  GX_HIP(hipGetDevice(&p->device));
+  plan_setup(p, d, ps);
+  std::vector<std::pair<DevBuf *, size_t>> bufs;
+  plan_buffers(p, ps, bufs);
+  GX_CHECK((int)bufs.size() == kPlanBuffers);
+  char *q = (char *)ws;
+  for (auto &b : bufs) {
+    if (b.second) b.first->borrow(q);
+    q += (b.second + kPlanAlign - 1) / kPlanAlign * kPlanAlign;
+  }
+  GX_CHECK((size_t)(q - (char *)ws) <= bytes);
+  rc = plan_upload(p, s);
+  if (rc) return rc;
+  *out = guard.release();
+  return GX_OK;
+}
+
+}  // namespace gx
 
 extern "C" int gx_volume_plan_destroy(gx_volume_plan *plan) {
   delete plan;
@@ -949,3 +1033,10 @@ extern "C" int gx_synth_projections(const gx_synth_desc *d, float *d_out,
   GX_HIP(synth_projections(&a, d_out, (hipStream_t)stream));
   return GX_OK;
 }
+
+namespace gx {
+
+// The error channel for the library's other translation units.
+int fail_status(int code, const char *msg) { return fail(code, "%s", msg); }
+
+}  // namespace gx

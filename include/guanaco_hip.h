@@ -228,6 +228,73 @@ typedef struct gx_synth_desc {
 int gx_synth_projections(const gx_synth_desc *desc, float *d_out,
                          void *stream);
 
+/* ---- serving (no reference counterpart) ---------------------------------
+ * A request loop for whole-volume reconstructions of different sizes: the
+ * server owns one device allocation (its budget) and admits requests in
+ * arrival order as their footprint (gx_server_request_bytes: plan, input
+ * copy, volume) fits and one of its streams is free; with backfill a later
+ * request that fits may start while an earlier one waits. Requests run
+ * concurrently, one stream each, with no allocation or device-wide
+ * synchronisation; each result volume stays in server memory until
+ * released. Results do not depend on what else is running. */
+typedef struct gx_server gx_server;
+
+typedef struct gx_server_cfg {
+  size_t budget_bytes; /* device memory the server holds; 0 = 80% of free */
+  int max_inflight;    /* requests running at once; 0 = 4 */
+  int backfill;        /* 1: later requests that fit may start first */
+  int gpu_index;       /* -1 = current device */
+} gx_server_cfg;
+
+#define GX_REQ_QUEUED 0
+#define GX_REQ_RUNNING 1
+#define GX_REQ_DONE 2
+#define GX_REQ_FAILED 3
+#define GX_REQ_CANCELLED 4
+
+typedef struct gx_request_info {
+  int state;               /* GX_REQ_* */
+  double submit_s;         /* server clock (s since creation) */
+  double start_s, end_s;   /* admitted, finished */
+  double device_ms;        /* correction + filter + backprojection on device */
+  double updates;          /* X * Y * Z * T */
+  size_t footprint_bytes;  /* held while running */
+  const float *volume;     /* device, (Y, Z, W) float32, until released */
+  char error[256];         /* failure message */
+} gx_request_info;
+
+typedef struct gx_server_stats {
+  size_t submitted, completed, failed, cancelled, queued, running;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submission to completion */
+  double updates_per_s;   /* completed updates / (last end - first submit) */
+  double span_s;
+  size_t budget_bytes, used_bytes, peak_bytes;
+} gx_server_stats;
+
+int gx_server_create(const gx_server_cfg *cfg, gx_server **server);
+/* Waits for the running requests; requests still queued are cancelled. */
+int gx_server_destroy(gx_server *server);
+/* Device bytes a request holds: the plan and, for host input, its copy
+ * (while running), plus the volume (until released). */
+int gx_server_request_bytes(const gx_volume_desc *desc, int input_on_host,
+                            size_t *bytes);
+/* The descriptor and its arrays are copied; `input` (layout per
+ * desc->input_layout; host or device memory) must stay valid until the
+ * request has finished. Refused (GX_ENOMEM) when the footprint exceeds the
+ * budget. */
+int gx_server_submit(gx_server *server, const gx_volume_desc *desc,
+                     const float *input, int input_on_host, uint64_t *id);
+/* Blocks until the request has finished (or timeout_s > 0 passes). */
+int gx_server_wait(gx_server *server, uint64_t id, double timeout_s,
+                   gx_request_info *info);
+int gx_server_info(gx_server *server, uint64_t id, gx_request_info *info);
+/* A queued request is dropped; a running one is reported cancelled when it
+ * ends. */
+int gx_server_cancel(gx_server *server, uint64_t id);
+/* Frees a finished request's volume; the id is then unknown. */
+int gx_server_release(gx_server *server, uint64_t id);
+int gx_server_get_stats(gx_server *server, gx_server_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

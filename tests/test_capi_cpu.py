@@ -100,8 +100,11 @@ def test_plan_footprint_is_host_only_arithmetic():
     f16 = _capi.plan_bytes(num_pixels=1024, num_rows=1024, depth=256,
                            angles=angles, storage="f16")
     slab32 = 121 * 1024 * 1024 * 4
-    assert slab32 <= f32 < slab32 + (1 << 20)
-    assert slab32 // 2 <= f16 < slab32 // 2 + (1 << 20)
+    # the staged backprojector's window table: one int2 per (32 x 8 tile,
+    # tilt padded to 8)
+    win = (1024 // 32) * (256 // 8) * 128 * 8
+    assert slab32 + win <= f32 < slab32 + win + (1 << 16)
+    assert slab32 // 2 + win <= f16 < slab32 // 2 + win + (1 << 16)
     ctf = _capi.plan_bytes(num_pixels=1024, num_rows=1024, depth=256,
                            angles=angles, storage="f16", num_defocus=4,
                            defocus=numpy.linspace(1e4, 2e4, 4))
