@@ -253,11 +253,12 @@ SERVE_DESC = ("Poisson arrivals of whole-volume requests: detector S^2 with S "
               "+-60 degree series per size, device-resident), ndf uniform in "
               "1..16, df uniform in 10000..40000 A, Cs 2.7 mm, 300 kV, "
               "1.5 A/px, volume S x S/4 x S, FP16 slabs; arrival rate 0.7 x "
-              "the capacity measured closed-loop first")
+              "the capacity measured closed-loop first; 4 requests in "
+              "flight, footprint admission with bounded backfill")
 
 
 def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
-                  log=sys.stderr):
+                  backfill=True, log=sys.stderr):
     """Continuous batching through the native server (gx_server_*):
     capacity measured closed-loop on 24 requests, then n_requests Poisson
     arrivals at 0.7 x capacity; latency is submission to completion."""
@@ -295,7 +296,7 @@ def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
                   wavelength=l, Cs=2.7e7, phi=math.asin(0.1))
         return view, angles121[start:start + T], S // 4, kw
 
-    srv = Server(max_inflight=max_inflight, backfill=False)
+    srv = Server(max_inflight=max_inflight, backfill=backfill)
     try:
         # capacity: 24 requests submitted at once
         rng = numpy.random.default_rng(seed + 1)
@@ -310,10 +311,10 @@ def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
         span = max(i["end_s"] for i in infos) - min(i["submit_s"]
                                                     for i in infos)
         capacity = len(reqs) / span
-        print("bench: serving capacity %.1f requests/s (%d closed-loop)"
-              % (capacity, len(reqs)), file=log, flush=True)
-        srv.close()
-        srv = Server(max_inflight=max_inflight, backfill=False)
+        print("bench: serving capacity %.1f requests/s (%d closed-loop, "
+              "budget %.1f GB)" % (capacity, len(reqs),
+                                   srv.stats()["budget_bytes"] / 1e9),
+              file=log, flush=True)
         rate = 0.7 * capacity
         rng = numpy.random.default_rng(seed)
         work = [make(rng) for _ in range(n_requests)]
@@ -349,23 +350,28 @@ def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
         srv.close()
         for d in series.values():
             d.free()
-    lat = [i["latency_ms"] for i in done if i["state"] == "done"]
+    ok = [i for i in done if i["state"] == "done"]
+    lat = [i["latency_ms"] for i in ok]
+    span = (max(i["end_s"] for i in ok) - min(i["submit_s"] for i in ok)
+            if ok else 0.0)
     return {
         "description": SERVE_DESC,
         "requests": n_requests,
-        "completed": int(st["completed"]),
-        "failed": int(st["failed"]),
+        "completed": len(ok),
+        "failed": sum(1 for i in done if i["state"] == "failed"),
         "capacity_requests_per_s": capacity,
         "arrival_rate_per_s": rate,
         "p50_ms": percentile(lat, 50),
         "p99_ms": percentile(lat, 99),
         "mean_ms": sum(lat) / max(1, len(lat)),
         "max_ms": max(lat) if lat else None,
-        "device_ms_mean": sum(i["device_ms"] for i in done) / max(1, len(done)),
-        "updates_per_s": st["updates_per_s"],
+        "device_ms_mean": sum(i["device_ms"] for i in ok) / max(1, len(ok)),
+        "updates_per_s": (sum(i["updates"] for i in ok) / span
+                          if span > 0 else 0.0),
         "max_inflight": max_inflight,
         "budget_bytes": int(st["budget_bytes"]),
         "peak_bytes": int(st["peak_bytes"]),
+        "backfill": backfill,
     }
 
 

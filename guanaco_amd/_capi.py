@@ -283,6 +283,13 @@ class DeviceArray:
             pass
 
 
+def mem_free():
+    """Free device memory in bytes."""
+    f, tot = _sz(), _sz()
+    call("gx_mem_get_info", ctypes.byref(f), ctypes.byref(tot))
+    return f.value
+
+
 class DeviceView:
     """Non-owning view of device memory (e.g. a slice of a DeviceArray)."""
 

@@ -7,7 +7,9 @@
 //   bytes while it runs (plan, input copy) and `keep` bytes until the client
 //   releases its result (the volume). The head of the queue starts as soon as
 //   a slot (stream) is free and its footprint fits; with backfill, later
-//   requests that fit may start while the head waits. Requests larger than
+//   requests that fit may start while the head waits, but no request is
+//   overtaken more than kMaxBypass times (then nothing may start before it,
+//   so large requests cannot starve). Requests larger than
 //   the budget are refused at submission. Completions are found by polling
 //   the executor; several requests run at once, one per slot.
 // - latency statistics (submission to completion) with percentiles.
@@ -100,8 +102,11 @@ class Arena {
 enum class State { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3,
                    kCancelled = 4 };
 
+constexpr int kMaxBypass = 8;
+
 struct Job {
   uint64_t id = 0;
+  int bypassed = 0;  // later requests admitted before this one
   size_t work_bytes = 0, keep_bytes = 0;
   size_t work_off = 0, keep_off = 0;
   bool has_work = false, has_keep = false;
@@ -391,9 +396,13 @@ class Scheduler {
             }
             if (w) arena_.release(wo);
           }
-          if (!backfill_) break;
+          if (!backfill_ || c.bypassed >= kMaxBypass) break;
         }
         if (!j) return;
+        for (auto &q : queue_) {  // every request still ahead of j
+          if (q->id > j->id) break;
+          ++q->bypassed;
+        }
         j->state = State::kRunning;
         j->slot = slot;
         j->start_s = now();

@@ -124,6 +124,25 @@ static void fifo_and_backfill() {
   std::printf("OK fifo and backfill admission\n");
 }
 
+static void bounded_bypass() {
+  // a large request at the head is overtaken by small ones at most
+  // kMaxBypass times, then waits for the room it needs
+  FakeExec ex(64);
+  Scheduler<FakeExec> s(&ex, 4096, true);
+  const uint64_t small0 = submit(s, 2048, 0, req(1000000));  // holds half
+  const uint64_t big = submit(s, 4096, 0, req(1));
+  std::vector<uint64_t> smalls;
+  for (int i = 0; i < 20; ++i) smalls.push_back(submit(s, 256, 0, req(1)));
+  for (int i = 0; i < 40; ++i) s.step();
+  CHECK(state(s, small0) == State::kRunning);
+  CHECK(state(s, big) == State::kQueued);
+  int done = 0;
+  for (uint64_t id : smalls) done += state(s, id) == State::kDone;
+  CHECK(done == gx::serve::kMaxBypass);
+  CHECK(s.info(big)->bypassed == gx::serve::kMaxBypass);
+  std::printf("OK bounded backfill\n");
+}
+
 static void slots_limit() {
   FakeExec ex(2);
   Scheduler<FakeExec> s(&ex, 1 << 20, false);
@@ -234,6 +253,7 @@ static void threaded() {
 int main() {
   arena_checks();
   fifo_and_backfill();
+  bounded_bypass();
   slots_limit();
   refusal();
   cancel_checks();
