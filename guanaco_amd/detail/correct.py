@@ -3,16 +3,19 @@
 ``correct_projections`` keeps the reference's signature, units and return
 value ``(projections, min_defocus - defocus, max_defocus - defocus)``; with
 ``device="gpu"`` every tilt is corrected by the gfx950 ``corr`` kernels.
+``correct_file`` is the file driver behind the ``guanaco.correct`` command.
 """
 import tempfile
+import time
 from math import pi
 
 import numpy
 
 import guanaco_amd.detail as ext
+from guanaco_amd import mrc
 
-__all__ = ["correct_projections", "get_defocus_steps", "get_ctf",
-           "get_ctf_array"]
+__all__ = ["correct_projections", "correct_file", "get_defocus_steps",
+           "get_ctf", "get_ctf_array"]
 
 
 def get_centre(shape, centre, sinogram_order=True):
@@ -121,3 +124,75 @@ def correct_projections(projections, centre=None, pixel_size=1, energy=None,
     if ndf == 1:
         corrected = corrected.reshape(projections.shape)
     return corrected, mn - defocus, mx - defocus
+
+
+def correct_file(input_filename, output_filename, centre=None, energy=None,
+                 defocus=None, num_defocus=None, step_defocus=None,
+                 spherical_aberration=None, astigmatism=None,
+                 astigmatism_angle=None, phase_shift=None, device="cpu"):
+    """CTF-corrects an MRC tilt series into a (T, ndf, Y, X) MRC volume
+    stack (correct.py:321-473).
+
+    Angles come from the input's FEI extended header ("Alpha tilt",
+    degrees), the pixel size from its voxel size. The output carries an FEI1
+    extended header with one block per (tilt, defocus): "Alpha tilt" in
+    degrees and "Defocus" = the block's defocus relative to `defocus`
+    (min_defocus - defocus + i * step), which reconstruct_file reads back as
+    the slab range.
+    """
+    start_time = time.time()
+    dev = ext.eDevice(device) if isinstance(device, (str, int)) else device
+    if dev != ext.eDevice.gpu:
+        raise RuntimeError(
+            "guanaco_amd implements the GPU device (device='gpu'); the CPU "
+            "device is the reference's own FFTW implementation")
+    print("Reading %s" % input_filename)
+    with mrc.mmap(input_filename) as infile:
+        voxel_size = infile.voxel_size
+        print("Voxel size: ", voxel_size)
+        header = infile.indexed_extended_header
+        if header is None:
+            raise ValueError("the input has no FEI extended header with tilt "
+                             "angles")
+        assert infile.data.shape[0] == header.shape[0], \
+            "Inconsistent data/header array shape"
+        angles = numpy.deg2rad(numpy.asarray(header["Alpha tilt"],
+                                             numpy.float64)).astype(
+                                                 numpy.float32)
+        for i, a in enumerate(angles):
+            print("Image %d; angle %.4f deg" % (i + 1, numpy.rad2deg(a)))
+        assert voxel_size["x"] == voxel_size["y"]
+        pixel_size = float(voxel_size["x"])
+        projections = infile.data
+        centre = get_centre(projections.shape, centre, False)
+        if num_defocus is None or num_defocus == 0:
+            num_defocus = 1
+            step_defocus = 0
+        output_shape = (projections.shape[0], num_defocus,
+                        projections.shape[1], projections.shape[2])
+        extended_header = numpy.zeros(output_shape[0:2],
+                                      mrc.FEI1_EXTENDED_HEADER_DTYPE)
+        extended_header["Metadata size"] = extended_header.dtype.itemsize
+        print("Writing corrected projections to %s" % output_filename)
+        with mrc.new_mmap(output_filename, output_shape,
+                          mrc_mode=mrc.mode_from_dtype(numpy.float32),
+                          overwrite=True, extended_header=extended_header,
+                          exttyp=b"FEI1") as outfile:
+            outfile.voxel_size = voxel_size
+            _, min_defocus, max_defocus = correct_projections(
+                projections, centre, pixel_size, energy, defocus,
+                num_defocus, step_defocus, spherical_aberration or 0,
+                astigmatism or 0, astigmatism_angle or 0, phase_shift or 0,
+                outfile.data, device)
+            if step_defocus is None:
+                if num_defocus == 1:
+                    step_defocus = 0
+                else:
+                    step_defocus = (max_defocus - min_defocus) / (
+                        num_defocus - 1)
+            blocks = outfile.indexed_extended_header.reshape(
+                output_shape[0:2])
+            blocks["Alpha tilt"] = numpy.rad2deg(angles)[:, None]
+            blocks["Defocus"] = (min_defocus + numpy.arange(num_defocus)
+                                 * step_defocus)[None, :]
+    print("Time: %.2f seconds" % (time.time() - start_time))

@@ -4,16 +4,20 @@
 ``device="gpu"`` the slices go through the gfx950 path, batched into one
 device-resident call. ``reconstruct_projections`` is the fused path: raw
 projections in, CTF correction, filtering and backprojection on the device,
-without materialising the corrected stack on the host.
+without materialising the corrected stack on the host. ``reconstruct_file``
+is the file driver behind the ``guanaco`` command.
 """
+import time
+
 import numpy
 
 import guanaco_amd.detail as ext
+from guanaco_amd import mrc
 from guanaco_amd.detail import mp
-from guanaco_amd.detail.correct import get_defocus_steps
+from guanaco_amd.detail.correct import correct_projections, get_defocus_steps
 
-__all__ = ["reconstruct", "reconstruct_projections", "get_centre",
-           "compute_angular_weights", "get_weights"]
+__all__ = ["reconstruct", "reconstruct_projections", "reconstruct_file",
+           "get_centre", "compute_angular_weights", "get_weights"]
 
 
 def get_centre(shape, centre, sinogram_order=True):
@@ -157,3 +161,135 @@ def reconstruct_projections(projections, angles, centre=None, pixel_size=1,
                            weights=weights, storage=storage,
                            gpu_index=gpu_index, **kw)
     return out
+
+
+def _require_gpu(device):
+    dev = ext.eDevice(device) if isinstance(device, (str, int)) else device
+    if dev != ext.eDevice.gpu:
+        raise RuntimeError(
+            "guanaco_amd implements the GPU device (device='gpu'); the CPU "
+            "device is the reference's own FFTW implementation")
+
+
+def read_projection_metadata(infile):
+    """Tilt angles (rad, float32), voxel size and the relative defocus range
+    of a projection file (reconstruct.py:245-292): angles from the FEI
+    extended header's "Alpha tilt" (degrees); for a corrected (T, ndf, Y, X)
+    stack, the defocus range from "Defocus" of the first tilt's blocks."""
+    voxel_size = infile.voxel_size
+    print("Voxel size: ", voxel_size)
+    header = infile.indexed_extended_header
+    if header is None:
+        raise ValueError(
+            "the input has no FEI extended header with tilt angles; give "
+            "start_angle, step_angle and pixel_size")
+    shape = infile.data.shape
+    if len(shape) == 3:
+        assert shape[0] == header.shape[0], "Inconsistent data/header shape"
+        tilt = header["Alpha tilt"]
+        min_defocus = max_defocus = 0
+    else:
+        assert header.size == shape[0] * shape[1], \
+            "Inconsistent data/header shape"
+        header = header.reshape(shape[0:2])
+        tilt = header[:, 0]["Alpha tilt"]
+        defocus = numpy.asarray(header[0, :]["Defocus"], dtype=numpy.float32)
+    angles = numpy.deg2rad(numpy.asarray(tilt, numpy.float64)).astype(
+        numpy.float32)
+    for i, a in enumerate(angles):
+        print("Image %d; angle %.4f deg" % (i + 1, numpy.rad2deg(a)))
+    if len(shape) == 4:
+        for i, d in enumerate(defocus):
+            print("Correction %d; defocus %.4f A" % (i + 1, d))
+        min_defocus, max_defocus = numpy.min(defocus), numpy.max(defocus)
+    return angles, voxel_size, min_defocus, max_defocus
+
+
+def reconstruct_file(input_filename, output_filename,
+                     corrected_filename="GUANACO_CORRECTED.dat",
+                     start_angle=None, step_angle=None, pixel_size=None,
+                     centre=None, energy=None, defocus=None, num_defocus=None,
+                     step_defocus=None, spherical_aberration=None,
+                     astigmatism=None, astigmatism_angle=None,
+                     phase_shift=None, device="cpu", ncore=None,
+                     transform=None, angular_weights=False, chunk_size=None,
+                     method="FBP_CTF", num_iter=None, storage="f32",
+                     depth=None, fused=None):
+    """Reconstructs an MRC tilt series into an MRC volume
+    (reconstruct.py:214-402).
+
+    The input is a (T, Y, X) projection stack or a (T, ndf, Y, X) corrected
+    stack written by correct_file. Angles and pixel size come from the FEI
+    extended header and the voxel size unless start_angle, step_angle and
+    pixel_size are all given (degrees, Angstrom). With method "FBP_CTF" an
+    uncorrected stack is CTF-corrected first. The output is a float32
+    (Y, X, X) volume (Y, depth, X with `depth`) with the input's voxel size.
+
+    On the GPU an uncorrected stack takes the fused device path by default
+    (fused=None): correction, filtering and backprojection in one device
+    run, so the corrected intermediate `corrected_filename` is not written.
+    fused=False keeps the reference's two steps (the corrected stack in
+    `corrected_filename`, then reconstruct). Methods other than FBP_CTF
+    belong to tomopy in the reference and are not part of this package.
+    """
+    start_time = time.time()
+    _require_gpu(device)
+    if fused is None:
+        fused = True
+    print("Reading %s" % input_filename)
+    with mrc.mmap(input_filename) as infile:
+        projections = infile.data
+        assert projections.ndim in (3, 4), "Dimensions must be 3 or 4"
+        if start_angle is None or step_angle is None or pixel_size is None:
+            angles, voxel_size, min_defocus, max_defocus = \
+                read_projection_metadata(infile)
+            assert voxel_size["x"] == voxel_size["y"], \
+                "Pixel must be a square"
+            pixel_size = float(voxel_size["x"])
+        else:
+            angles = numpy.deg2rad(
+                start_angle + numpy.arange(projections.shape[0]) * step_angle)
+            voxel_size = pixel_size
+            min_defocus = max_defocus = 0
+        print("Pixel size %d A" % pixel_size)
+        X = projections.shape[-1]
+        output_shape = (projections.shape[-2], X if depth is None else depth,
+                        X)
+        centre = get_centre(projections.shape, centre, False)
+        corrected = projections.ndim == 4
+        if corrected:
+            method = "FBP_CTF"
+        if method != "FBP_CTF":
+            raise NotImplementedError(
+                "method %s is the reference's tomopy path; this package "
+                "implements FBP_CTF" % method)
+        run_fused = fused and not corrected and defocus is not None
+        if not corrected and defocus is not None and not run_fused:
+            projections, min_defocus, max_defocus = correct_projections(
+                projections, centre, pixel_size, energy, defocus,
+                num_defocus, step_defocus, spherical_aberration or 0,
+                astigmatism or 0, astigmatism_angle or 0, phase_shift or 0,
+                corrected_filename, device)
+        print("Writing reconstruction to %s" % output_filename)
+        with mrc.new_mmap(output_filename, output_shape, mrc_mode=2,
+                          overwrite=True) as outfile:
+            outfile.voxel_size = voxel_size
+            if run_fused:
+                reconstruct_projections(
+                    projections, angles, centre=centre,
+                    pixel_size=pixel_size, energy=energy, defocus=defocus,
+                    num_defocus=num_defocus, step_defocus=step_defocus,
+                    spherical_aberration=spherical_aberration or 0,
+                    astigmatism=astigmatism or 0,
+                    astigmatism_angle=astigmatism_angle or 0,
+                    phase_shift=phase_shift or 0, transform=transform,
+                    angular_weights=angular_weights, depth=depth,
+                    storage=storage, reconstruction=outfile.data)
+            else:
+                reconstruct(projections, angles, outfile.data, centre=centre,
+                            pixel_size=pixel_size, min_defocus=min_defocus,
+                            max_defocus=max_defocus, sinogram_order=False,
+                            transform=transform,
+                            angular_weights=angular_weights, device=device,
+                            ncore=ncore, storage=storage, depth=depth)
+    print("Time: %.2f seconds" % (time.time() - start_time))
