@@ -4,12 +4,18 @@
 
 One step reconstructs one whole tilt series that is already resident in HBM:
 CTF phase-flip correction (2D R2C, per-defocus flip, C2R), ramp filter and
-backprojection of every defocus slab into the volume. Two paths run on the
-same input:
+backprojection of every defocus slab into the volume. Three paths run on
+the same input:
 
 - plain:   FP32 storage, exact reference arithmetic, kernels launched eagerly;
 - batched: FP16 storage of the filtered slabs, FP32 accumulation, the whole
-           request captured in one HIP graph and replayed.
+           request captured in one HIP graph and replayed;
+- served:  the batched request submitted repeatedly to the native serving
+           loop, two requests on the device at once (their kernels overlap).
+
+Before timing, each plan's backprojection tile (rows per tile, staged or
+direct kernel) is chosen by the autotuner (gx_volume_plan_autotune) on the
+plan's own data; the candidates' timings are reported.
 
 The value is the batched path's whole-job throughput in voxel*tilt updates/s
 (updates = X * Y * Z * T; ndf does not multiply it). rel-L2 of both volumes
@@ -74,7 +80,7 @@ def percentile(a, q):
 
 # ---------------------------------------------------------------- GPU -----
 
-def make_plans(C, w, angles, storage):
+def plan_kwargs(w, storage):
     from oracle import oracle as O  # only for the wavelength constant
     l = O.electron_wavelength(300000)
     kw = dict(num_defocus=w["ndf"], pixel_size=w["ps"], centre=w["X"] / 2.0,
@@ -86,7 +92,44 @@ def make_plans(C, w, angles, storage):
         dfs = numpy.array([mn + d * st for d in range(n)], numpy.float64)
         kw.update(min_defocus=mn - w["df"], max_defocus=mx - w["df"],
                   defocus=dfs, wavelength=l, Cs=2.7e7, phi=w["phi"])
-    return C.VolumePlan(w["X"], w["Y"], angles, w["Z"], **kw)
+    return kw
+
+
+def make_plans(C, w, angles, storage):
+    return C.VolumePlan(w["X"], w["Y"], angles, w["Z"],
+                        **plan_kwargs(w, storage))
+
+
+def served_path(C, w, angles, d_in, storage, n, max_inflight, log):
+    """The same request n times through the native server (gx_server_*),
+    max_inflight on the device at once: each request's correction, filter
+    and backprojection run on its own stream and overlap with the others'.
+    Throughput is n * updates over the wall time from the first submission
+    to the last completion (after 2 warm-up requests)."""
+    from guanaco_amd.serve import Server
+    kw = plan_kwargs(w, storage)
+    kw.pop("centre")
+    with Server(max_inflight=max_inflight) as srv:
+        for r in [srv.submit(d_in, angles, w["Z"], **kw) for _ in range(2)]:
+            r.wait()
+            r.release()
+        t0 = time.perf_counter()
+        reqs = [srv.submit(d_in, angles, w["Z"], **kw) for _ in range(n)]
+        infos = []
+        for r in reqs:
+            infos.append(r.wait())
+            r.release()
+        wall = time.perf_counter() - t0
+    assert all(i["state"] == "done" for i in infos), infos
+    updates = float(w["X"]) * w["Y"] * w["Z"] * w["T"]
+    lat = [i["latency_ms"] for i in infos]
+    out = dict(storage=storage, requests=n, max_inflight=max_inflight,
+               value=n * updates / wall, ms_per_request=1e3 * wall / n,
+               p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+               device_ms_mean=sum(i["device_ms"] for i in infos) / n)
+    print("bench: served (%s, %d in flight) %.2f ms/request" % (
+        storage, max_inflight, out["ms_per_request"]), file=log, flush=True)
+    return out
 
 
 def _defocus_steps(df, ndf, width, centre, ps):
@@ -402,6 +445,10 @@ def main():
                     help="requests of the serving phase (default 64 after "
                          "the c1-c3 workloads, 256 for c5; 0: skip)")
     ap.add_argument("--storage", default="f16", choices=["f16", "bf16"])
+    ap.add_argument("--served", type=int, default=8,
+                    help="requests of the served path (the workload n times "
+                         "through the server; 0: skip)")
+    ap.add_argument("--served-inflight", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -489,8 +536,12 @@ def main():
         if args.only and args.only != path:
             continue
         plan = make_plans(C, w, angles, storage)
+        tuned = None
         if args.bp:
             plan.set_backprojector(args.bp)
+        else:
+            # the tile (rows per tile, staged or direct) by measurement
+            tuned = plan.autotune(d_in, vol, stream)
         kind, slots = plan.backprojector()
         plans[path] = plan
         ms, lat, wall = time_path(C, plan, d_in, vol, stream, args.steps,
@@ -499,12 +550,8 @@ def main():
         bp_ms = time_kernel(C, lambda: plan.backproject(vol, stream), stream,
                             max(3, args.steps))
         other = {}
-        if kind == C.BP_STAGED:
-            # the direct kernel on the same slabs, for comparison
-            plan.set_backprojector("direct")
-            other["bp_direct_ms"] = time_kernel(
-                C, lambda: plan.backproject(vol, stream), stream, 3)
-            plan.set_backprojector("staged")
+        if tuned is not None:
+            other["autotune"] = tuned
         plan.run(d_in, vol, stream)
         stream.synchronize()
         results[path] = dict(storage=storage, ms_per_step=ms,
@@ -515,10 +562,16 @@ def main():
                              bp_updates_per_s=updates / (bp_ms * 1e-3),
                              backprojector=("staged, %d slab(s) per tilt"
                                             % slots if kind == C.BP_STAGED
-                                            else "direct"), **other)
+                                            else "direct"),
+                             tile_rows=plan.tile()[0], **other)
         print("bench: %s (%s) %.2f ms/step, BP %.2f ms, filter %.2f ms"
               % (path, storage, ms, bp_ms, filt_ms), file=sys.stderr,
               flush=True)
+
+    if args.served > 0 and args.only is None:
+        results["served"] = served_path(C, w, angles, d_in, args.storage,
+                                        args.served, args.served_inflight,
+                                        sys.stderr)
 
     head = results.get("batched") or results["plain"]
     s_bytes = 2 if head["storage"] in ("f16", "bf16") else 4
@@ -576,6 +629,9 @@ def main():
     if "batched" in results and "plain" in results:
         line["speedup_batched_vs_plain"] = (results["batched"]["value"] /
                                             results["plain"]["value"])
+    if "served" in results and "plain" in results:
+        line["speedup_served_vs_plain"] = (results["served"]["value"] /
+                                           results["plain"]["value"])
     if check:
         rows = numpy.linspace(0, Y - 1, 4).astype(int)
         proj = d_in.to_host()

@@ -110,6 +110,19 @@ class ServerStats(ctypes.Structure):
                 ("used_bytes", _sz), ("peak_bytes", _sz)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("kind", _int), ("rows", _int), ("filter_ms", ctypes.c_float),
+                ("bp_ms", ctypes.c_float)]
+
+
+TUNE_MAX = 8
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("kind", _int), ("rows", _int), ("n", _int),
+                ("cand", TuneEntry * TUNE_MAX)]
+
+
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
     "gx_last_error": [],
@@ -158,6 +171,14 @@ SIGNATURES = {
                                          ctypes.POINTER(_int)],
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
+    "gx_volume_plan_autotune": [_vp, _vp, _vp, _vp,
+                                ctypes.POINTER(TuneResult)],
+    "gx_volume_plan_get_tile": [_vp, ctypes.POINTER(_int),
+                                ctypes.POINTER(_int)],
+    "gx_volume_plan_set_tile": [_vp, _int, _int],
+    "gx_tune_save": [ctypes.c_char_p],
+    "gx_tune_load": [ctypes.c_char_p],
+    "gx_tune_clear": [],
     "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],
     "gx_server_create": [ctypes.POINTER(ServerCfg), ctypes.POINTER(_vp)],
     "gx_server_destroy": [_vp],
@@ -428,6 +449,34 @@ class VolumePlan:
              ctypes.byref(n))
         return k.value, n.value
 
+    def tile(self):
+        """(rows per backprojection tile, backprojector kind)."""
+        r, k = _int(), _int()
+        call("gx_volume_plan_get_tile", self.handle, ctypes.byref(r),
+             ctypes.byref(k))
+        return r.value, k.value
+
+    def set_tile(self, rows, kind):
+        """One of the autotuner's candidates, by hand (takes effect with the
+        next filter pass)."""
+        call("gx_volume_plan_set_tile", self.handle, int(rows),
+             BACKPROJECTOR[kind] if isinstance(kind, str) else kind)
+
+    def autotune(self, d_in, d_vol, stream=None):
+        """Times every tile candidate on this plan's data and keeps the
+        fastest (remembered for later plans of this shape); d_vol holds the
+        reconstruction afterwards. Returns the choice and the timings."""
+        r = TuneResult()
+        call("gx_volume_plan_autotune", self.handle, d_in.ptr, d_vol.ptr,
+             stream.handle if isinstance(stream, Stream) else stream,
+             ctypes.byref(r))
+        names = {BP_DIRECT: "direct", BP_STAGED: "staged"}
+        return {"backprojector": names[r.kind], "rows": r.rows,
+                "candidates": [
+                    {"backprojector": names[c.kind], "rows": c.rows,
+                     "filter_ms": round(c.filter_ms, 4),
+                     "bp_ms": round(c.bp_ms, 4)} for c in r.cand[:r.n]]}
+
     def slabs(self):
         p = _vp()
         n = _sz()
@@ -446,6 +495,20 @@ class VolumePlan:
             self.destroy()
         except Exception:
             pass
+
+
+def tune_save(path):
+    """Writes the remembered tile choices to `path` (text)."""
+    call("gx_tune_save", os.fsencode(path))
+
+
+def tune_load(path):
+    """Merges the tile choices in `path` into the remembered ones."""
+    call("gx_tune_load", os.fsencode(path))
+
+
+def tune_clear():
+    call("gx_tune_clear")
 
 
 def plan_bytes(**kw):

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -615,6 +616,85 @@ struct gx_volume_plan {
   double wavelength = 0, Cs = 0, Ca = 0, Pa = 0, phi = 0, pixel_size = 0;
 };
 
+// ------------------------------------------------------------ autotuning --
+//
+// Tile choices per (device model, plan shape), measured by
+// gx_volume_plan_autotune. The candidates never exceed the row blocking a
+// plan's slab buffer is sized for (16 rows for float32 slabs, 32 for 16-bit
+// ones), so a choice applies to a plan without reallocation.
+
+namespace {
+
+struct TuneChoice {
+  int kind, rows;
+};
+
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune;
+
+// "gfx950:256": architecture and compute units of the current device
+std::string device_model() {
+  static std::mutex mu;
+  static std::string names[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return "?";
+  std::lock_guard<std::mutex> lock(mu);
+  if (names[dev].empty()) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "?";
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    names[dev] = arch + ":" + std::to_string(prop.multiProcessorCount);
+  }
+  return names[dev];
+}
+
+std::string tune_key(const gx_volume_plan *p) {
+  char b[256];
+  std::snprintf(b, sizeof b, "%s %d %d %d %d %d %d %d %d %d %d",
+                device_model().c_str(), p->X, p->Y, p->T, p->ndf, p->Z, p->W,
+                p->storage, p->layout, p->ctf ? 1 : 0, p->bp_staged);
+  return b;
+}
+
+// the largest candidate comes first: it is the default and sizes the slabs
+void tile_candidates(int storage, int rows[2]) {
+  rows[0] = storage == GX_STORE_F32 ? 16 : 32;
+  rows[1] = rows[0] / 2;
+}
+
+bool tile_valid(const gx_volume_plan *p, int rows, int kind) {
+  int c[2];
+  tile_candidates(p->storage, c);
+  if (rows != c[0] && rows != c[1]) return false;
+  if (kind == GX_BP_STAGED) return p->bp_staged != 0;
+  return kind == GX_BP_DIRECT;
+}
+
+void set_tile(gx_volume_plan *p, int rows, int kind) {
+  p->YB = rows;
+  p->NYB = (p->Y + rows - 1) / rows;
+  p->bp_kind = kind;
+}
+
+void apply_tuned(gx_volume_plan *p) {
+  const std::string key = tune_key(p);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end() && tile_valid(p, it->second.rows, it->second.kind))
+    set_tile(p, it->second.rows, it->second.kind);
+}
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+};
+
+}  // namespace
+
 namespace {
 
 constexpr size_t kPlanAlign = 256;
@@ -725,6 +805,7 @@ void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
                        d->max_defocus, &p->dscale, &p->doffset);
   p->bp_staged = bp_slots(d->num_defocus, p->dscale);
   p->bp_kind = p->bp_staged ? GX_BP_STAGED : GX_BP_DIRECT;
+  apply_tuned(p);
   const size_t T = d->num_angles;
   p->h_coef.assign(4 * (size_t)bp_tilt_pad((int)T), 0.0f);
   tilt_geometry(T, d->angles, d->centre, p->h_coef.data());
@@ -988,6 +1069,138 @@ extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
   if (d_slab) *d_slab = p->slab.p;
   if (bytes) *bytes = p->slab_bytes;
   if (rows_per_block) *rows_per_block = p->YB;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_get_tile(gx_volume_plan *p, int *rows,
+                                       int *kind) {
+  GX_CHECK(p != nullptr);
+  if (rows) *rows = p->YB;
+  if (kind) *kind = p->bp_kind;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_set_tile(gx_volume_plan *p, int rows,
+                                       int kind) {
+  GX_CHECK(p != nullptr);
+  if (!tile_valid(p, rows, kind))
+    return fail(GX_EUNSUPPORTED,
+                "tile of %d rows with backprojector %d is not a candidate for "
+                "this plan", rows, kind);
+  set_tile(p, rows, kind);
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_autotune(gx_volume_plan *p, const float *d_in,
+                                       float *d_vol, void *stream,
+                                       gx_tune_result *out) {
+  GX_CHECK(p != nullptr && d_in != nullptr && d_vol != nullptr);
+  GX_HIP(hipSetDevice(p->device));
+  hipStream_t s = (hipStream_t)stream;
+  EventPair ev;
+  GX_HIP(hipEventCreate(&ev.a));
+  GX_HIP(hipEventCreate(&ev.b));
+  gx_tune_result r{};
+  int rows[2];
+  tile_candidates(p->storage, rows);
+  const int kinds[2] = {GX_BP_STAGED, GX_BP_DIRECT};
+  float best = 0.0f;
+  int rc;
+  for (int ri = 0; ri < 2; ++ri) {
+    set_tile(p, rows[ri], p->bp_kind);
+    // the first pass warms up; the second is timed
+    if ((rc = gx_volume_filter(p, d_in, s))) return rc;
+    GX_HIP(hipEventRecord(ev.a, s));
+    if ((rc = gx_volume_filter(p, d_in, s))) return rc;
+    GX_HIP(hipEventRecord(ev.b, s));
+    GX_HIP(hipEventSynchronize(ev.b));
+    float fms = 0.0f;
+    GX_HIP(hipEventElapsedTime(&fms, ev.a, ev.b));
+    for (int kind : kinds) {
+      if (!tile_valid(p, rows[ri], kind)) continue;
+      p->bp_kind = kind;
+      if ((rc = gx_volume_backproject(p, d_vol, s))) return rc;
+      GX_HIP(hipEventRecord(ev.a, s));
+      for (int i = 0; i < 2; ++i)
+        if ((rc = gx_volume_backproject(p, d_vol, s))) return rc;
+      GX_HIP(hipEventRecord(ev.b, s));
+      GX_HIP(hipEventSynchronize(ev.b));
+      float bms = 0.0f;
+      GX_HIP(hipEventElapsedTime(&bms, ev.a, ev.b));
+      bms *= 0.5f;
+      GX_CHECK(r.n < GX_TUNE_MAX);
+      r.cand[r.n++] = gx_tune_entry{kind, rows[ri], fms, bms};
+      if (r.n == 1 || fms + bms < best) {
+        best = fms + bms;
+        r.kind = kind;
+        r.rows = rows[ri];
+      }
+    }
+  }
+  set_tile(p, r.rows, r.kind);
+  if ((rc = gx_volume_run(p, d_in, d_vol, s))) return rc;
+  GX_HIP(hipStreamSynchronize(s));
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[tune_key(p)] = TuneChoice{r.kind, r.rows};
+  }
+  if (out) *out = r;
+  return GX_OK;
+}
+
+extern "C" int gx_tune_save(const char *path) {
+  GX_CHECK(path != nullptr);
+  std::string text =
+      "# guanaco_amd tile choices: model X Y T ndf Z W storage layout ctf "
+      "slabs -> kind rows\n";
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const auto &e : g_tune)
+      text += e.first + " " + std::to_string(e.second.kind) + " " +
+              std::to_string(e.second.rows) + "\n";
+  }
+  FILE *f = std::fopen(path, "w");
+  if (!f) return fail(GX_EINVAL, "cannot write %s", path);
+  const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+  if (std::fclose(f) != 0 || !ok) return fail(GX_EINVAL, "cannot write %s", path);
+  return GX_OK;
+}
+
+extern "C" int gx_tune_load(const char *path) {
+  GX_CHECK(path != nullptr);
+  FILE *f = std::fopen(path, "r");
+  if (!f) return fail(GX_EINVAL, "cannot read %s", path);
+  std::map<std::string, TuneChoice> got;
+  char line[512];
+  int lineno = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    ++lineno;
+    std::string l(line);
+    while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+    if (l.empty() || l[0] == '#') continue;
+    // key (11 fields) then kind and rows
+    const size_t p2 = l.find_last_of(' ');
+    const size_t p1 = p2 == std::string::npos ? p2 : l.find_last_of(' ', p2 - 1);
+    int kind = -1, rows = -1, nfield = 1;
+    for (char c : l) nfield += c == ' ';
+    if (p1 == std::string::npos || nfield != 13 ||
+        std::sscanf(l.c_str() + p1, "%d %d", &kind, &rows) != 2 ||
+        (kind != GX_BP_DIRECT && kind != GX_BP_STAGED) ||
+        (rows != 8 && rows != 16 && rows != 32)) {
+      std::fclose(f);
+      return fail(GX_EINVAL, "%s:%d: not a tile choice line", path, lineno);
+    }
+    got[l.substr(0, p1)] = TuneChoice{kind, rows};
+  }
+  std::fclose(f);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  for (const auto &e : got) g_tune[e.first] = e.second;
+  return GX_OK;
+}
+
+extern "C" int gx_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
   return GX_OK;
 }
 

@@ -271,10 +271,41 @@ __device__ __forceinline__ float inv_norm(const uint32_t *tmax, int t,
   return n > 0.0f ? 1.0f / n : 1.0f;
 }
 
-// Two (ya, yb) row pairs per workgroup; L = 2X threads-per-pair layout shared
-// by the X-point inverse (end radix EL/2) and the L-point filter (end radix
-// EL).
-template <int X, int STORE>
+// Two values of one detector column from rows (y, y + 1), packed as they lie
+// in the slab (row y first).
+template <int STORE>
+struct RowPair;
+template <>
+struct RowPair<kF32> {
+  using T = float2;
+  __device__ __forceinline__ static T pack(float a, float b) {
+    return make_float2(a, b);
+  }
+};
+template <>
+struct RowPair<kF16> {
+  using T = uint32_t;
+  __device__ __forceinline__ static T pack(float a, float b) {
+    return (uint32_t)__half_as_ushort(__float2half(a)) |
+           ((uint32_t)__half_as_ushort(__float2half(b)) << 16);
+  }
+};
+template <>
+struct RowPair<kBF16> {
+  using T = uint32_t;
+  __device__ __forceinline__ static T pack(float a, float b) {
+    return f32_to_bf16_bits(a) | (f32_to_bf16_bits(b) << 16);
+  }
+};
+
+// One workgroup filters RB = 4 * IT consecutive rows of one (tilt, defocus)
+// spectrum, RB dividing the slab's row blocking: two pair slots (L = 2X
+// threads-per-pair layout shared by the X-point inverse, end radix EL/2, and
+// the L-point filter, end radix EL) each take IT row pairs in turn and keep
+// their outputs in registers, so that each thread finally writes the 2 * IT
+// consecutive rows of its columns as whole 16-byte stores into the
+// [y / YB][x][y % YB] slab layout.
+template <int X, int STORE, int IT>
 __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
     k_rows_c2r_filter(const float2 *__restrict__ G, int Y, int XB, int ndf,
                       int t0, int T, const float *__restrict__ wscale,
@@ -286,48 +317,76 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   constexpr int EX = EL / 2;
   constexpr int NT = L / EL;
   static_assert(X / EX == NT, "thread layouts must agree");
+  using PT = typename RowPair<STORE>::T;
+  constexpr bool kWide = (IT * sizeof(PT)) % 16 == 0;  // whole 16-byte stores
   __shared__ float2 s[2][fft_lds_len(L, EL)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
-  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const int r0 = blockIdx.x * (4 * IT) + p * (2 * IT);  // first row of slot
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
 
-  // the L-point twiddles serve both L-point transforms; loaded first so that
-  // their latency hides behind the spectrum loads and the X-point inverse
+  // the L-point twiddles serve both L-point transforms of every row pair
   typename Fft<L, EL, -1, 1>::Twiddles twl;
   twl.load(tid, tw);
-  float2 v[EX];
-#pragma unroll
-  for (int m = 0; m < EX; ++m)
-    v[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
-  Fft<X, EX, 1, 1>::run(v, s[p], tid, tw);  // v = a + i b (unnormalised)
-#pragma unroll
-  for (int m = 0; m < EX; ++m) s[p][tid + m * NT] = v[m];
-  lds_barrier();
-  float2 u[EL];
-#pragma unroll
-  for (int m = 0; m < EL; ++m)
-    u[m] = m < EL / 2 ? s[p][tid + m * NT] : make_float2(0.0f, 0.0f);
-  lds_barrier();
-  Fft<L, EL, -1, 1>::run(u, s[p], tid, twl);
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
   // filter.cpp:30-45: |j| / X on the L = 2X grid, times the 1/L of the
   // unnormalised inverse; X and L are powers of two, so j / (X L) is exact
   // and equals the reference's (j / X) * (1 / L)
   constexpr float kRamp = 1.0f / ((float)X * (float)L);
+  PT out[EL / 2][IT];
 #pragma unroll
-  for (int m = 0; m < EL; ++m) {
-    const int j = tid + m * NT;
-    const float r = (float)(j <= X ? j : L - j) * kRamp;
-    u[m] = make_float2(u[m].x * r, u[m].y * r);
+  for (int it = 0; it < IT; ++it) {
+    const int ya = r0 + 2 * it, yb = ya + 1;
+    float2 v[EX];
+#pragma unroll
+    for (int m = 0; m < EX; ++m)
+      v[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+    Fft<X, EX, 1, 1>::run(v, s[p], tid, tw);  // v = a + i b (unnormalised)
+#pragma unroll
+    for (int m = 0; m < EX; ++m) s[p][tid + m * NT] = v[m];
+    lds_barrier();
+    float2 u[EL];
+#pragma unroll
+    for (int m = 0; m < EL; ++m)
+      u[m] = m < EL / 2 ? s[p][tid + m * NT] : make_float2(0.0f, 0.0f);
+    lds_barrier();
+    Fft<L, EL, -1, 1>::run(u, s[p], tid, twl);
+#pragma unroll
+    for (int m = 0; m < EL; ++m) {
+      const int j = tid + m * NT;
+      const float r = (float)(j <= X ? j : L - j) * kRamp;
+      u[m] = make_float2(u[m].x * r, u[m].y * r);
+    }
+    Fft<L, EL, 1, 1>::run(u, s[p], tid, twl);
+#pragma unroll
+    for (int m = 0; m < EL / 2; ++m)
+      out[m][it] = RowPair<STORE>::pack(u[m].x * sc, u[m].y * sc);
   }
-  Fft<L, EL, 1, 1>::run(u, s[p], tid, twl);
-  const float sc = wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
-  using ST = StoreT<STORE>;
+  using ST = typename StoreT<STORE>::T;
+  if (r0 >= Y) return;
+  const bool whole = r0 + 2 * IT <= Y;
 #pragma unroll
   for (int m = 0; m < EL / 2; ++m) {
     const int n = tid + m * NT;
-    if (ya < Y) slab[slab_index(k, t, ya, n, T, NYB, X, YB)] = ST::pack(u[m].x * sc);
-    if (yb < Y) slab[slab_index(k, t, yb, n, T, NYB, X, YB)] = ST::pack(u[m].y * sc);
+    ST *dst = slab + slab_index(k, t, r0, n, T, NYB, X, YB);
+    if (whole && kWide) {
+      uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+      for (int q = 0; q < (int)(IT * sizeof(PT) / 16); ++q) {
+        uint4 w;
+        __builtin_memcpy(&w, reinterpret_cast<const char *>(out[m]) + 16 * q,
+                         16);
+        d4[q] = w;
+      }
+    } else if (whole) {
+      PT *dp = reinterpret_cast<PT *>(dst);
+#pragma unroll
+      for (int it = 0; it < IT; ++it) dp[it] = out[m][it];
+    } else {
+      const ST *src = reinterpret_cast<const ST *>(out[m]);
+      for (int r = 0; r < 2 * IT && r0 + r < Y; ++r) dst[r] = src[r];
+    }
   }
 }
 
@@ -490,6 +549,49 @@ hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
   return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s);
 }
 
+// Rows per workgroup: the slab's row blocking YB, at most 16 (float32) or
+// 32 (16-bit) rows; 4 for 4096-point rows, whose 1024-thread workgroups
+// leave no registers for held outputs.
+inline int c2r_filter_rows(int X, int store, int YB) {
+  if (X >= 4096) return 4;
+  const int cap = store == kF32 ? 16 : 32;
+  return YB < cap ? YB : cap;
+}
+
+template <int STORE, int IT>
+static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
+                                       int ndf, int t0, int ntilts, int T,
+                                       const float *wscale,
+                                       const uint32_t *tmax, float nrm_mul,
+                                       void *slab, int NYB, int YB,
+                                       hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  dim3 grid((Y + 4 * IT - 1) / (4 * IT), ndf, ntilts);
+  switch (X) {
+#define C_(n)                                                                  \
+  case n:                                                                      \
+    hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE, IT>), grid,                \
+                       dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
+                       ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
+                       tw);                                                    \
+    break;
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+#undef C_
+    case 4096:
+      if constexpr (IT == 1) {
+        hipLaunchKernelGGL((k_rows_c2r_filter<4096, STORE, 1>), grid,
+                           dim3(2 * (2 * 4096 / default_e(2 * 4096))), 0, s,
+                           G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul,
+                           (ST *)slab, NYB, YB, tw);
+        break;
+      }
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <int STORE>
 static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       int ndf, int t0, int ntilts, int T,
@@ -497,23 +599,19 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
                                       hipStream_t s, const float2 *tw) {
-  using ST = typename StoreT<STORE>::T;
-  dim3 grid((Y + 3) / 4, ndf, ntilts);
-  switch (X) {
-#define C_(n)                                                                  \
-  case n:                                                                      \
-    hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE>), grid,                    \
-                       dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
-                       ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
-                       tw);                                                    \
-    break;
-    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
-    C_(4096)
-#undef C_
-    default:
-      return hipErrorInvalidValue;
+  const int rb = c2r_filter_rows(X, STORE, YB);
+  if (YB % rb != 0) return hipErrorInvalidValue;
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+  if (rb == 4) return launch_c2r_filter_it<STORE, 1>(A_);
+  if constexpr (STORE == kF32) {
+    if (rb == 8) return launch_c2r_filter_it<STORE, 2>(A_);
+    if (rb == 16) return launch_c2r_filter_it<STORE, 4>(A_);
+  } else {
+    if (rb == 16) return launch_c2r_filter_it<STORE, 4>(A_);
+    if (rb == 32) return launch_c2r_filter_it<STORE, 8>(A_);
   }
-  return hipGetLastError();
+#undef A_
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,

@@ -193,6 +193,46 @@ int gx_volume_plan_get_backprojector(gx_volume_plan *plan, int *kind,
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);
 
+/* ---- autotuning (no reference counterpart) --------------------------------
+ * The backprojection tile of a plan shape is chosen by measurement. A tile
+ * is 32 x 8 voxels in (x, z) by `rows` detector rows in y, one voxel column
+ * per thread; `rows` is also the row blocking of the filtered slabs. The
+ * candidates are rows 8 and 16 for float32 slabs, 16 and 32 for float16 and
+ * bfloat16 ones, each with the staged and the direct backprojector (staged
+ * only where a tile spans at most 4 defocus slabs). gx_volume_plan_autotune
+ * filters d_in once per row count, times that filter pass and every
+ * backprojector on the plan's own slabs with HIP events on `stream`, keeps
+ * the candidate with the least filter + backprojection time, and leaves
+ * d_vol reconstructed with it. Every candidate computes the same volume.
+ * The choice is remembered per (device model, plan shape) for the life of
+ * the process, and plans of that shape created later start with it. The
+ * call synchronizes `stream`, cannot be captured into a graph, and must not
+ * overlap other calls on the same plan. */
+#define GX_TUNE_MAX 8
+typedef struct gx_tune_entry {
+  int kind;         /* GX_BP_* */
+  int rows;         /* rows per tile */
+  float filter_ms;  /* filter pass with this row blocking */
+  float bp_ms;      /* backprojection */
+} gx_tune_entry;
+typedef struct gx_tune_result {
+  int kind, rows;   /* the choice */
+  int n;            /* candidates timed */
+  gx_tune_entry cand[GX_TUNE_MAX];
+} gx_tune_result;
+int gx_volume_plan_autotune(gx_volume_plan *plan, const float *d_in,
+                            float *d_vol, void *stream, gx_tune_result *out);
+/* The plan's current tile (rows per tile and backprojector), and setting
+ * it to one of the candidates by hand; a new row count takes effect with
+ * the next filter pass (gx_volume_run, gx_volume_filter). */
+int gx_volume_plan_get_tile(gx_volume_plan *plan, int *rows, int *kind);
+int gx_volume_plan_set_tile(gx_volume_plan *plan, int rows, int kind);
+/* The remembered choices: written as text, one shape per line, read back
+ * (merged into the current ones), or forgotten. */
+int gx_tune_save(const char *path);
+int gx_tune_load(const char *path);
+int gx_tune_clear(void);
+
 /* ---- seeded synthetic workloads (no reference counterpart) --------------- */
 
 /* Tilt series of a random-ellipsoid phantom projected analytically in the

@@ -112,3 +112,27 @@ def test_plan_footprint_is_host_only_arithmetic():
     with pytest.raises(_capi.GxError):
         _capi.plan_bytes(num_pixels=1000, num_rows=1024, depth=256,
                          angles=angles, defocus=[1e4])
+
+
+def test_tile_choice_file_round_trip_and_errors(tmp_path):
+    from guanaco_amd import _capi as C
+    C.tune_clear()
+    line = "gfx950:256 1024 1024 121 1 256 1024 1 0 1 1 1 32"
+    good = tmp_path / "tiles.txt"
+    good.write_text("# a comment\n\n" + line + "\n")
+    C.tune_load(str(good))
+    out = tmp_path / "saved.txt"
+    C.tune_save(str(out))
+    kept = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert kept == [line]
+    bad = tmp_path / "bad.txt"
+    for text in ("gfx950:256 1 2 3 1 7\n", line.replace(" 32", " 12") + "\n",
+                 line[:-4] + " 5 32\n"):
+        bad.write_text(text)
+        with pytest.raises(C.GxError, match="not a tile choice"):
+            C.tune_load(str(bad))
+    with pytest.raises(C.GxError, match="cannot read"):
+        C.tune_load(str(tmp_path / "missing.txt"))
+    C.tune_clear()
+    C.tune_save(str(out))
+    assert all(l.startswith("#") for l in out.read_text().splitlines())

@@ -390,3 +390,61 @@ def test_staged_bp_equals_direct_reduced_storage(capi, case, storage):
     assert numpy.isfinite(out["staged"]).all()
     numpy.testing.assert_array_equal(out["staged"], out["direct"])
     plan.destroy()
+
+
+# ------------------------------------------------------------- autotuning
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("case", [BP_KIND_CASES[0], BP_KIND_CASES[2],
+                                  BP_KIND_CASES[4]])
+def test_autotuner_candidates_agree_and_choice_is_remembered(
+        capi, case, storage, tmp_path):
+    X, Y, T, ndf, Z, W = case[:6]
+    C.tune_clear()
+    plan, sino, angles = sinogram_plan(case, storage)
+    default = plan.tile()
+    d_in = C.DeviceArray.from_host(sino)
+    ref = C.DeviceArray((Y, Z, W))
+    plan.run(d_in, ref)
+    C.call("gx_device_synchronize")
+    ref = ref.to_host()
+    assert numpy.isfinite(ref).all()
+    # every candidate computes the same volume, bit for bit
+    big = 16 if storage == "f32" else 32
+    slots = plan.backprojector()[1]
+    kinds = ["staged", "direct"] if slots else ["direct"]
+    for rows in (big, big // 2):
+        for k in kinds:
+            plan.set_tile(rows, k)
+            d_vol = C.DeviceArray((Y, Z, W))
+            C.call("gx_fill_u32", d_vol.ptr, 0x7fc00000, Y * Z * W, None)
+            plan.run(d_in, d_vol)
+            C.call("gx_device_synchronize")
+            numpy.testing.assert_array_equal(d_vol.to_host(), ref,
+                                             err_msg="%d %s" % (rows, k))
+    with pytest.raises(C.GxError, match="not a candidate"):
+        plan.set_tile(big * 2, "direct")
+    d_vol = C.DeviceArray((Y, Z, W))
+    res = plan.autotune(d_in, d_vol)
+    numpy.testing.assert_array_equal(d_vol.to_host(), ref)
+    assert len(res["candidates"]) == 2 * len(kinds)
+    best = min(res["candidates"], key=lambda c: c["filter_ms"] + c["bp_ms"])
+    assert (res["rows"], res["backprojector"]) == (best["rows"],
+                                                   best["backprojector"])
+    chosen = (res["rows"], C.BACKPROJECTOR[res["backprojector"]])
+    assert plan.tile() == chosen
+    # later plans of this shape start with the choice; others do not
+    p2, _, _ = sinogram_plan(case, storage)
+    assert p2.tile() == chosen
+    p2.destroy()
+    C.tune_save(str(tmp_path / "tiles.txt"))
+    C.tune_clear()
+    p3, _, _ = sinogram_plan(case, storage)
+    assert p3.tile() == default
+    p3.destroy()
+    C.tune_load(str(tmp_path / "tiles.txt"))
+    p4, _, _ = sinogram_plan(case, storage)
+    assert p4.tile() == chosen
+    p4.destroy()
+    plan.destroy()
+    C.tune_clear()
