@@ -195,6 +195,29 @@ def time_kernel(C, fn, stream, reps):
     return ms
 
 
+def host_path(w, angles, proj, storage, reps=3):
+    """The drop-in call with host buffers (guanaco_amd.reconstruct_projections
+    on a numpy tilt series, numpy volume returned): PCIe copies, plan set-up
+    and the device path, as a reference user's call sees it."""
+    import guanaco_amd as G
+    vol = numpy.empty((w["Y"], w["Z"], w["X"]), numpy.float32)
+    kw = dict(pixel_size=w["ps"], depth=w["Z"], storage=storage,
+              reconstruction=vol)
+    if w["df"] is not None:
+        kw.update(energy=300, defocus=w["df"], num_defocus=w["ndf"],
+                  spherical_aberration=2.7,
+                  phase_shift=math.degrees(w["phi"]))
+    G.reconstruct_projections(proj, angles, **kw)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        G.reconstruct_projections(proj, angles, **kw)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    updates = float(w["X"]) * w["Y"] * w["Z"] * w["T"]
+    return dict(storage=storage, ms_per_request=ms,
+                value=updates / (ms * 1e-3),
+                bytes_in=int(proj.nbytes), bytes_out=int(vol.nbytes))
+
+
 # ---------------------------------------------------- CPU oracle legs -----
 
 def _oracle_correct_rows(args):
@@ -451,6 +474,8 @@ def main():
     ap.add_argument("--served-inflight", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) measurement")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--dump", default=None,
                     help="directory to save the arrays the oracle check "
@@ -652,6 +677,12 @@ def main():
         print("bench: serving %s" % {k: v for k, v in line["serving"].items()
                                      if k != "description"},
               file=sys.stderr, flush=True)
+    if args.only is None and not args.no_host:
+        proj = d_in.to_host()
+        line["host_buffers"] = host_path(w, angles, proj, args.storage)
+        print("bench: host buffers (%s) %.1f ms/request incl. PCIe" % (
+            args.storage, line["host_buffers"]["ms_per_request"]),
+            file=sys.stderr, flush=True)
     if not args.no_cpu:
         proj = d_in.to_host()
         line["cpu_baseline"] = cpu_baseline(w, proj, angles, args.cpu_budget)
