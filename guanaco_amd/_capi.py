@@ -110,6 +110,17 @@ class ServerStats(ctypes.Structure):
                 ("used_bytes", _sz), ("peak_bytes", _sz)]
 
 
+class CtfDesc(ctypes.Structure):
+    _fields_ = [("width", _sz), ("height", _sz), ("num_defocus", _sz),
+                ("pixel_size", ctypes.c_double),
+                ("defocus", ctypes.POINTER(ctypes.c_double)),
+                ("wavelength", ctypes.c_double), ("Cs", ctypes.c_double),
+                ("Ca", ctypes.c_double), ("Pa", ctypes.c_double),
+                ("dd", ctypes.c_double), ("theta_c", ctypes.c_double),
+                ("phi", ctypes.c_double), ("envelope", _int),
+                ("recentre", _int), ("gpu_index", _int)]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("kind", _int), ("rows", _int), ("filter_ms", ctypes.c_float),
                 ("bp_ms", ctypes.c_float)]
@@ -180,6 +191,7 @@ SIGNATURES = {
     "gx_tune_load": [ctypes.c_char_p],
     "gx_tune_clear": [],
     "gx_synth_projections": [ctypes.POINTER(SynthDesc), _vp, _vp],
+    "gx_ctf_grid": [ctypes.POINTER(CtfDesc), _vp],
     "gx_server_create": [ctypes.POINTER(ServerCfg), ctypes.POINTER(_vp)],
     "gx_server_destroy": [_vp],
     "gx_server_request_bytes": [ctypes.POINTER(VolumeDesc), _int,
@@ -495,6 +507,26 @@ class VolumePlan:
             self.destroy()
         except Exception:
             pass
+
+
+def ctf_grid(width, height, pixel_size, defocus, wavelength, Cs=0.0, Ca=0.0,
+             Pa=0.0, dd=0.0, theta_c=0.0, phi=0.0, envelope=True,
+             recentre=False, out=None, gpu_index=-1):
+    """complex64 (ndf, height, width) CTF images computed on the device
+    (gx_ctf_grid); units as the reference's CTF: A and rad."""
+    df = numpy.ascontiguousarray(numpy.atleast_1d(defocus), numpy.float64)
+    shape = (df.size, int(height), int(width))
+    if out is None:
+        out = numpy.empty(shape, numpy.complex64)
+    assert out.shape == shape and out.dtype == numpy.complex64 \
+        and out.flags.c_contiguous and out.flags.writeable
+    d = CtfDesc(int(width), int(height), df.size, float(pixel_size),
+                df.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                float(wavelength), float(Cs), float(Ca), float(Pa), float(dd),
+                float(theta_c), float(phi), 1 if envelope else 0,
+                1 if recentre else 0, int(gpu_index))
+    call("gx_ctf_grid", ctypes.byref(d), out.ctypes.data)
+    return out
 
 
 def tune_save(path):

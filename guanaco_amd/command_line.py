@@ -1,10 +1,12 @@
-"""The ``guanaco`` and ``guanaco.correct`` commands
-(reference src/guanaco/command_line.py; console scripts at setup.py:64-69).
+"""The ``guanaco``, ``guanaco.correct``, ``guanaco.plot_ctf`` and
+``guanaco.generate_ctf`` commands (reference src/guanaco/command_line.py;
+console scripts at setup.py:64-69).
 
 Same options, destinations and defaults as the reference, so the same
 command lines work: ``python -m guanaco_amd -i tilts.mrc -o rec.mrc -d gpu
 --df 20000 --ndf 8`` reconstructs, ``python -m guanaco_amd.correct -i ...``
-corrects. The device option keeps the reference's spellings (``-d gpu``;
+corrects, ``python -m guanaco_amd.plot_ctf`` and ``python -m
+guanaco_amd.generate_ctf --size 512,512`` are the CTF tools. The device option keeps the reference's spellings (``-d gpu``;
 the reference declares it as the single option string "-d,--device", which
 argparse matches by prefix) and also takes ``--device``. Two options are
 additive: ``--storage`` (filtered-projection precision on the GPU) and
@@ -12,10 +14,11 @@ additive: ``--storage`` (filtered-projection precision on the GPU) and
 """
 import argparse
 
+from guanaco_amd import plot
 from guanaco_amd.detail.correct import correct_file
 from guanaco_amd.detail.reconstruct import reconstruct_file
 
-__all__ = ["add_argument", "main", "correct"]
+__all__ = ["add_argument", "main", "correct", "plot_ctf", "generate_ctf"]
 
 
 def add_argument(parser, name):
@@ -171,6 +174,71 @@ def main(args=None):
         depth=args.depth,
         fused=args.fused,
     )
+
+
+_ENVELOPE_ARGS = ("energy", "defocus", "spherical_aberration", "astigmatism",
+                  "astigmatism_angle", "phase_shift", "defocus_spread",
+                  "source_spread", "chromatic_aberration", "energy_spread",
+                  "acceleration_voltage_spread", "current_spread")
+
+
+def _envelope_kwargs(args):
+    return {k: getattr(args, k) for k in _ENVELOPE_ARGS}
+
+
+def plot_ctf_parser():
+    parser = argparse.ArgumentParser(description="Plot the CTF")
+    parser.add_argument("-o", dest="output", default="ctf.png",
+                        help="The output image file for the CTF plot")
+    parser.add_argument("--q_max", dest="q_max", type=float, default=0.5,
+                        help="The maximum spatial frequency (1/A)")
+    parser.add_argument("--component", dest="component", type=str,
+                        choices=["real", "imag", "abs"], default="imag",
+                        help="The component to plot")
+    parser.add_argument("--envelope", dest="envelope", type=str,
+                        choices=["spatial", "temporal", "all"], default=None,
+                        help="Plot the CTF envelope")
+    for name in _ENVELOPE_ARGS:
+        add_argument(parser, name)
+    return parser
+
+
+def plot_ctf(args=None):
+    """The ``guanaco.plot_ctf`` command (command_line.py:329-409)."""
+    args = plot_ctf_parser().parse_args(args=args)
+    plot.plot_ctf(filename=args.output, q_max=args.q_max,
+                  component=args.component, envelope=args.envelope,
+                  **_envelope_kwargs(args))
+
+
+def generate_ctf_parser():
+    parser = argparse.ArgumentParser(description="Generate the CTF image")
+    parser.add_argument("-o", dest="output", default="ctf.mrc",
+                        help="The output image file for the CTF image")
+    parser.add_argument("--size", "--image_size", dest="image_size",
+                        type=lambda x: list(map(int, x.split(","))),
+                        default=None, required=True, help="The image size")
+    parser.add_argument("--px", "--pixel_size", dest="pixel_size", type=float,
+                        default=1, help="The pixel size")
+    # type=bool as the reference: any non-empty value switches it on
+    parser.add_argument("--recentre", dest="recentre", type=bool,
+                        default=False,
+                        help="Recentre the CTF in Fourier space")
+    for name in _ENVELOPE_ARGS[:2] + ("num_defocus", "step_defocus") + \
+            _ENVELOPE_ARGS[2:]:
+        add_argument(parser, name)
+    return parser
+
+
+def generate_ctf(args=None):
+    """The ``guanaco.generate_ctf`` command (command_line.py:412-494); the
+    images are computed on the GPU."""
+    args = generate_ctf_parser().parse_args(args=args)
+    plot.generate_ctf(filename=args.output, image_size=args.image_size,
+                      pixel_size=args.pixel_size,
+                      num_defocus=args.num_defocus,
+                      step_defocus=args.step_defocus,
+                      recentre=args.recentre, **_envelope_kwargs(args))
 
 
 def correct_parser():

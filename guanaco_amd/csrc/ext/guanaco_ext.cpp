@@ -91,6 +91,77 @@ py::array_t<std::complex<double>> ctf_simple_q(const guanaco::CTF &c,
   return out;
 }
 
+using F64Array = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+// ext/ctf.h:78-129: the CTF functions mapped over arrays of (q, theta)
+template <typename T, typename F>
+py::array_t<T> map_q_theta(const F64Array &q, const F64Array &theta, F f) {
+  GUANACO_ASSERT(q.size() == theta.size());
+  std::vector<py::ssize_t> shape(q.shape(), q.shape() + q.ndim());
+  py::array_t<T> out(shape);
+  T *o = out.mutable_data();
+  const double *qp = q.data(), *tp = theta.data();
+  for (py::ssize_t i = 0; i < q.size(); ++i) o[i] = f(qp[i], tp[i]);
+  return out;
+}
+
+py::array_t<double> Es_q(const guanaco::CTF &c, const F64Array &q,
+                         const F64Array &theta) {
+  return map_q_theta<double>(q, theta, [&](double a, double t) {
+    return std::exp(guanaco::get_log_Es(c, a, t));
+  });
+}
+
+py::array_t<double> Et_q(const guanaco::CTF &c, const F64Array &q) {
+  return map_q_theta<double>(
+      q, q, [&](double a, double) { return std::exp(guanaco::get_log_Et(c, a)); });
+}
+
+py::array_t<std::complex<double>> ctf_q(const guanaco::CTF &c,
+                                        const F64Array &q,
+                                        const F64Array &theta) {
+  return map_q_theta<std::complex<double>>(
+      q, theta, [&](double a, double t) { return guanaco::get_ctf(c, a, t); });
+}
+
+py::array_t<double> ctf_simple_real_q(const guanaco::CTF &c,
+                                      const F64Array &q,
+                                      const F64Array &theta) {
+  return map_q_theta<double>(q, theta, [&](double a, double t) {
+    return guanaco::get_ctf_simple_real(c, a, t);
+  });
+}
+
+py::array_t<double> ctf_simple_imag_q(const guanaco::CTF &c,
+                                      const F64Array &q,
+                                      const F64Array &theta) {
+  return map_q_theta<double>(q, theta, [&](double a, double t) {
+    return guanaco::get_ctf_simple_imag(c, a, t);
+  });
+}
+
+// ext/ctf.h:131-169: the FFT-ordered (h, w) grids
+py::array_t<std::complex<double>> ctf_wh(const guanaco::CTF &c, std::size_t w,
+                                         std::size_t h, double ps) {
+  py::array_t<std::complex<double>> out({h, w});
+  guanaco::get_ctf_n(c, out.mutable_data(), w, h, ps);
+  return out;
+}
+
+py::array_t<double> ctf_simple_real_wh(const guanaco::CTF &c, std::size_t w,
+                                       std::size_t h, double ps) {
+  py::array_t<double> out({h, w});
+  guanaco::get_ctf_n_simple_real(c, out.mutable_data(), w, h, ps);
+  return out;
+}
+
+py::array_t<double> ctf_simple_imag_wh(const guanaco::CTF &c, std::size_t w,
+                                       std::size_t h, double ps) {
+  py::array_t<double> out({h, w});
+  guanaco::get_ctf_n_simple_imag(c, out.mutable_data(), w, h, ps);
+  return out;
+}
+
 py::tuple q_and_theta(std::size_t w, std::size_t h, double ps) {
   py::array_t<double> q({h, w}), a({h, w});
   GUANACO_ASSERT(ps > 0);
@@ -210,9 +281,17 @@ PYBIND11_MODULE(guanaco_ext, m) {
   py::implicitly_convertible<py::str, eDevice>();
   py::implicitly_convertible<py::int_, eDevice>();
 
-  // ext/ctf.h:173-279 (the subset the correction path uses)
+  // ext/ctf.h:172-279
   m.def("get_electron_wavelength", &guanaco::get_electron_wavelength,
         py::arg("V"));
+  m.def("get_defocus_spread", &guanaco::get_defocus_spread, py::arg("Cc"),
+        py::arg("dEE"), py::arg("dII"), py::arg("dVV"));
+  m.def("q_to_Q", &guanaco::q_to_Q, py::arg("q"), py::arg("l"), py::arg("Cs"));
+  m.def("Q_to_q", &guanaco::Q_to_q, py::arg("Q"), py::arg("l"), py::arg("Cs"));
+  m.def("df_to_D", &guanaco::df_to_D, py::arg("df"), py::arg("l"),
+        py::arg("Cs"));
+  m.def("D_to_df", &guanaco::D_to_df, py::arg("D"), py::arg("l"),
+        py::arg("Cs"));
   m.def("get_q_and_theta", &q_and_theta, py::arg("w"), py::arg("h"),
         py::arg("ps"));
   py::class_<guanaco::CTF>(m, "CTF")
@@ -241,9 +320,21 @@ PYBIND11_MODULE(guanaco_ext, m) {
       .def_readwrite("theta_c", &guanaco::CTF::theta_c)
       .def_readwrite("phi", &guanaco::CTF::phi)
       .def("get_chi", &guanaco::get_chi, py::arg("q"), py::arg("theta"))
+      .def("get_Es", &Es_q, py::arg("q"), py::arg("theta"))
+      .def("get_Et", &Et_q, py::arg("q"))
+      .def("get_ctf", &ctf_q, py::arg("q"), py::arg("theta"))
       .def("get_ctf_simple", &ctf_simple_q, py::arg("q"), py::arg("theta"))
+      .def("get_ctf_simple_real", &ctf_simple_real_q, py::arg("q"),
+           py::arg("theta"))
+      .def("get_ctf_simple_imag", &ctf_simple_imag_q, py::arg("q"),
+           py::arg("theta"))
+      .def("get_ctf", &ctf_wh, py::arg("w"), py::arg("h"), py::arg("ps"))
       .def("get_ctf_simple", &ctf_simple_wh, py::arg("w"), py::arg("h"),
-           py::arg("ps"));
+           py::arg("ps"))
+      .def("get_ctf_simple_real", &ctf_simple_real_wh, py::arg("w"),
+           py::arg("h"), py::arg("ps"))
+      .def("get_ctf_simple_imag", &ctf_simple_imag_wh, py::arg("w"),
+           py::arg("h"), py::arg("ps"));
 
   // ext/correct.h:71-76
   m.def("corr", &corr, py::arg("image"), py::arg("ctf"), py::arg("rec"),

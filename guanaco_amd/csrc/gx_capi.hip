@@ -1204,6 +1204,42 @@ extern "C" int gx_tune_clear(void) {
   return GX_OK;
 }
 
+// ------------------------------------------------------- CTF images -------
+
+extern "C" int gx_ctf_grid(const gx_ctf_desc *d, float *out) {
+  GX_CHECK(d != nullptr && out != nullptr && d->defocus != nullptr);
+  GX_CHECK(d->width > 0 && d->height > 0 && d->num_defocus > 0);
+  GX_CHECK(d->pixel_size > 0);
+  GX_CHECK(d->width <= (1u << 16) && d->height <= (1u << 16));
+  for (size_t k = 0; k < d->num_defocus; ++k)
+    if (!std::isfinite(d->defocus[k]))
+      return fail(GX_EINVAL, "defocus %zu is not finite", k);
+  int rc = select_device(d->gpu_index);
+  if (rc) return rc;
+  const size_t n = d->width * d->height;
+  // defoci per launch: about 256 MiB of output, at least one
+  size_t batch = ((size_t)256 << 20) / (n * sizeof(float2));
+  if (batch < 1) batch = 1;
+  if (batch > d->num_defocus) batch = d->num_defocus;
+  GX_CHECK(batch <= 65535);
+  DevBuf d_out, d_df;
+  GX_HIP(d_out.alloc(batch * n * sizeof(float2)));
+  GX_HIP(d_df.alloc(d->num_defocus * sizeof(double)));
+  GX_HIP(hipMemcpy(d_df.p, d->defocus, d->num_defocus * sizeof(double),
+                   hipMemcpyHostToDevice));
+  const CtfParams c{d->wavelength, d->Cs, d->Ca, d->Pa,
+                    d->dd, d->theta_c, d->phi};
+  for (size_t k0 = 0; k0 < d->num_defocus; k0 += batch) {
+    const size_t nk = std::min(batch, d->num_defocus - k0);
+    GX_HIP(launch_ctf_grid(d_out.as<float2>(), (int)d->width, (int)d->height,
+                           d->pixel_size, d_df.as<double>() + k0, (int)nk, c,
+                           d->envelope != 0, d->recentre != 0, nullptr));
+    GX_HIP(hipMemcpy(out + 2 * k0 * n, d_out.p, nk * n * sizeof(float2),
+                     hipMemcpyDeviceToHost));
+  }
+  return GX_OK;
+}
+
 // ------------------------------------------------------- synthetic data ---
 
 extern "C" int gx_synth_projections(const gx_synth_desc *d, float *d_out,

@@ -148,6 +148,67 @@ hipError_t launch_mask_from_ctf(int8_t *mask, const float2 *ctf, int X, int Y,
   return hipGetLastError();
 }
 
+// ------------------------------------------------------ CTF generation ---
+
+// The CTF on the FFT-ordered grid (ctf.h:466-478 get_ctf_n, ctf.h:489-503
+// get_ctf_n_simple), FP64 throughout, stored as complex64: with the
+// envelopes (ctf.h:188-223, 250-258) A exp(log Es + log Et) exp(-i (chi -
+// phi)), without them exp(-i (chi - phi)). recentre puts the zero frequency
+// at (h/2, w/2), as numpy.fft.fftshift does. blockIdx.y = defocus.
+template <bool ENVELOPE>
+__global__ void k_ctf_grid(float2 *__restrict__ out, int w, int h,
+                           double inv_ps, const double *__restrict__ defocus,
+                           CtfParams c, int recentre) {
+  const size_t n = (size_t)w * h;
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int x = (int)(e % w), y = (int)(e / w);
+  const size_t w2 = (size_t)w / 2, h2 = (size_t)h / 2;
+  const double dx =
+      ((double)(((size_t)x + w2) % (size_t)w) - (double)w2) / (double)w;
+  const double dy =
+      ((double)(((size_t)y + h2) % (size_t)h) - (double)h2) / (double)h;
+  const double q = sqrt((dx * dx + dy * dy) * inv_ps * inv_ps);
+  const double theta = atan2(dy, dx);
+  const double df = defocus[blockIdx.y] + c.Ca * cos(2 * (theta - c.Pa));
+  // the reference's operation order (ctf.h:234-238, 189-223)
+  const double chi =
+      M_PI * (c.Cs * c.l * c.l * c.l * q * q * q * q / 2 - c.l * df * q * q);
+  double mag = 1.0;
+  if constexpr (ENVELOPE) {
+    const double u =
+        1 + 2 * M_PI * M_PI * c.theta_c * c.theta_c * c.dd * c.dd * q * q;
+    const double v = c.Cs * c.l * c.l * c.l * q * q * q - df * c.l * q;
+    const double log_es =
+        -(M_PI * M_PI * c.theta_c * c.theta_c * v * v / (c.l * c.l * u));
+    const double log_et =
+        -(M_PI * M_PI * c.l * c.l * c.dd * c.dd * q * q * q * q / (2 * u));
+    mag = (1 / sqrt(u)) * exp(log_es + log_et);
+  }
+  double sn, cs;
+  sincos(-(chi - c.phi), &sn, &cs);
+  const int ox = recentre ? (int)(((size_t)x + w2) % (size_t)w) : x;
+  const int oy = recentre ? (int)(((size_t)y + h2) % (size_t)h) : y;
+  out[(size_t)blockIdx.y * n + (size_t)oy * w + ox] =
+      make_float2((float)(mag * cs), (float)(mag * sn));
+}
+
+hipError_t launch_ctf_grid(float2 *out, int w, int h, double ps,
+                           const double *d_defocus, int ndf,
+                           const CtfParams &c, bool envelope, bool recentre,
+                           hipStream_t s) {
+  const size_t n = (size_t)w * h;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)ndf);
+  if (envelope) {
+    hipLaunchKernelGGL(k_ctf_grid<true>, grid, dim3(256), 0, s, out, w, h,
+                       1.0 / ps, d_defocus, c, recentre ? 1 : 0);
+  } else {
+    hipLaunchKernelGGL(k_ctf_grid<false>, grid, dim3(256), 0, s, out, w, h,
+                       1.0 / ps, d_defocus, c, recentre ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
 // ------------------------------------------------- pass 1: rows, R2C -------
 
 // Four image rows per workgroup (two complex pairs). Writes the half

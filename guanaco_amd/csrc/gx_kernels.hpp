@@ -38,6 +38,14 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, size_t n,
                            hipStream_t s);
 
+// CTF parameters in the reference's units (ctf.h:167-178): A and rad
+struct CtfParams {
+  double l, Cs, Ca, Pa, dd, theta_c, phi;
+};
+hipError_t launch_ctf_grid(float2 *out, int w, int h, double ps,
+                           const double *d_defocus, int ndf,
+                           const CtfParams &c, bool envelope, bool recentre,
+                           hipStream_t s);
 hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
                            const double *d_defocus, double l, double Cs,
                            double Ca, double Pa, double phi, double ps,

@@ -132,6 +132,89 @@ void get_ctf_n_simple(const CTF &c, std::complex<double> *ctf, std::size_t w,
   }
 }
 
+double get_defocus_spread(double Cc, double dEE, double dII, double dVV) {
+  return Cc * std::sqrt(dEE * dEE + (2 * dII) * (2 * dII) + dVV * dVV);
+}
+
+double q_to_Q(double q, double l, double Cs) {
+  return q * std::pow(Cs * l * l * l, 1.0 / 4.0);
+}
+
+double Q_to_q(double Q, double l, double Cs) {
+  return Q / std::pow(Cs * l * l * l, 1.0 / 4.0);
+}
+
+double df_to_D(double df, double l, double Cs) { return df / std::sqrt(Cs * l); }
+
+double D_to_df(double D, double l, double Cs) { return D * std::sqrt(Cs * l); }
+
+namespace {
+// 1 + 2 pi^2 theta_c^2 dd^2 q^2, shared by the envelopes and their scale
+double envelope_u(const CTF &c, double q) {
+  return 1 + 2 * kPi * kPi * c.theta_c * c.theta_c * c.dd * c.dd * q * q;
+}
+}  // namespace
+
+double get_log_Es(const CTF &c, double q, double theta) {
+  const double df = c.df + c.Ca * std::cos(2 * (theta - c.Pa));
+  const double u = envelope_u(c, q);
+  const double v = c.Cs * c.l * c.l * c.l * q * q * q - df * c.l * q;
+  return -(kPi * kPi * c.theta_c * c.theta_c * v * v / (c.l * c.l * u));
+}
+
+double get_log_Et(const CTF &c, double q) {
+  const double u = envelope_u(c, q);
+  return -(kPi * kPi * c.l * c.l * c.dd * c.dd * q * q * q * q / (2 * u));
+}
+
+double get_A(const CTF &c, double q) { return 1 / std::sqrt(envelope_u(c, q)); }
+
+std::complex<double> get_ctf(const CTF &c, double q, double theta) {
+  const double chi = get_chi(c, q, theta);
+  const double e = get_log_Es(c, q, theta) + get_log_Et(c, q);
+  return get_A(c, q) * std::exp(std::complex<double>(e, -(chi - c.phi)));
+}
+
+double get_ctf_simple_real(const CTF &c, double q, double theta) {
+  return std::cos(-(get_chi(c, q, theta) - c.phi));
+}
+
+double get_ctf_simple_imag(const CTF &c, double q, double theta) {
+  return std::sin(-(get_chi(c, q, theta) - c.phi));
+}
+
+namespace {
+template <typename F>
+void for_grid(std::size_t w, std::size_t h, double ps, F f) {
+  GUANACO_ASSERT(ps > 0);
+  const double inv_ps = 1.0 / ps;
+  for (std::size_t j = 0; j < h; ++j) {
+    for (std::size_t i = 0; i < w; ++i) {
+      double r, theta;
+      get_r_and_theta(i, j, w, h, inv_ps, &r, &theta);
+      f(r, theta);
+    }
+  }
+}
+}  // namespace
+
+void get_ctf_n(const CTF &c, std::complex<double> *ctf, std::size_t w,
+               std::size_t h, double ps) {
+  for_grid(w, h, ps, [&](double r, double t) { *ctf++ = get_ctf(c, r, t); });
+}
+
+void get_ctf_n_simple_real(const CTF &c, double *ctf, std::size_t w,
+                           std::size_t h, double ps) {
+  for_grid(w, h, ps,
+           [&](double r, double t) { *ctf++ = get_ctf_simple_real(c, r, t); });
+}
+
+void get_ctf_n_simple_imag(const CTF &c, double *ctf, std::size_t w,
+                           std::size_t h, double ps) {
+  for_grid(w, h, ps,
+           [&](double r, double t) { *ctf++ = get_ctf_simple_imag(c, r, t); });
+}
+
 // ---- whole-volume reconstruction --------------------------------------------
 
 namespace {

@@ -97,6 +97,28 @@ std::complex<double> get_ctf_simple(const CTF &c, double q,
                                     double theta);      // ctf.h:269-273
 void get_ctf_n_simple(const CTF &c, std::complex<double> *ctf, std::size_t w,
                       std::size_t h, double ps);        // ctf.h:489-503
+// ctf.h:61-64: Cc * sqrt(dEE^2 + (2 dII)^2 + dVV^2)
+double get_defocus_spread(double Cc, double dEE, double dII, double dVV);
+// ctf.h:114-162: dimensionless frequency and defocus (De Graef 10.10-10.11)
+double q_to_Q(double q, double l, double Cs);
+double Q_to_q(double Q, double l, double Cs);
+double df_to_D(double df, double l, double Cs);
+double D_to_df(double D, double l, double Cs);
+// ctf.h:188-223, 250-258: log envelopes (De Graef 10.53), scale and the full
+// CTF A exp(log Es + log Et) exp(-i (chi - phi))
+double get_log_Es(const CTF &c, double q, double theta);
+double get_log_Et(const CTF &c, double q);
+double get_A(const CTF &c, double q);
+std::complex<double> get_ctf(const CTF &c, double q, double theta);
+double get_ctf_simple_real(const CTF &c, double q, double theta);  // :285-288
+double get_ctf_simple_imag(const CTF &c, double q, double theta);  // :300-303
+// ctf.h:466-478, 516-541, 543-561: the FFT-ordered w x h grids
+void get_ctf_n(const CTF &c, std::complex<double> *ctf, std::size_t w,
+               std::size_t h, double ps);
+void get_ctf_n_simple_real(const CTF &c, double *ctf, std::size_t w,
+                           std::size_t h, double ps);
+void get_ctf_n_simple_imag(const CTF &c, double *ctf, std::size_t w,
+                           std::size_t h, double ps);
 
 // ---- whole-volume reconstruction (no reference counterpart) ----------------
 

@@ -119,6 +119,31 @@ int gx_filter(const float *rows, float *out, size_t nrows, size_t num_pixels,
 int gx_corr(const float *image, const float *ctf, float *rec, size_t xsize,
             size_t ysize, size_t num_ctf, int gpu_index);
 
+/* CTF images on the FFT-ordered grid, computed on the device in FP64 and
+ * stored as complex64: out[k][y][x] (interleaved re, im) for the absolute
+ * defocus defocus[k]. With envelope, CTF.get_ctf(w, h, ps) of the reference
+ * (ctf.h:466-478: A exp(log Es + log Et) exp(-i (chi - phi)), envelopes of
+ * ctf.h:188-223); without, CTF.get_ctf_simple(w, h, ps) (ctf.h:489-503).
+ * recentre moves the zero frequency to (height/2, width/2) (numpy fftshift),
+ * as generate_ctf(recentre=True) (plot.py:280-281). `out` is host memory of
+ * num_defocus * height * width * 2 floats. Synchronous. */
+typedef struct gx_ctf_desc {
+  size_t width, height, num_defocus;
+  double pixel_size;     /* A */
+  const double *defocus; /* host, num_defocus, A */
+  double wavelength;     /* A */
+  double Cs;             /* A */
+  double Ca;             /* A */
+  double Pa;             /* rad */
+  double dd;             /* defocus spread, A */
+  double theta_c;        /* source spread, rad */
+  double phi;            /* rad */
+  int envelope;          /* 1: with envelopes */
+  int recentre;          /* 1: zero frequency at the centre */
+  int gpu_index;         /* -1 = current device */
+} gx_ctf_desc;
+int gx_ctf_grid(const gx_ctf_desc *desc, float *out);
+
 /* ---- device-resident whole-volume path ------------------------------------ */
 
 /* Input layouts for gx_volume_run. */

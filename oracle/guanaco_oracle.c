@@ -92,6 +92,49 @@ void or_ctf_simple_grid(double l, double df, double Cs, double Ca, double Pa,
   }
 }
 
+/* ctf.h:188-200 (spatial incoherence, De Graef eq. 10.53), 204-214 (temporal)
+ * and 218-223 (scale), with u = 1 + 2 pi^2 theta_c^2 dd^2 q^2. */
+double or_log_es(double l, double df, double Cs, double Ca, double Pa,
+                 double dd, double theta_c, double q, double theta) {
+  double d = df + Ca * cos(2 * (theta - Pa));
+  double u = 1 + 2 * k_pi * k_pi * theta_c * theta_c * dd * dd * q * q;
+  double v = Cs * l * l * l * q * q * q - d * l * q;
+  return -(k_pi * k_pi * theta_c * theta_c * v * v / (l * l * u));
+}
+
+double or_log_et(double l, double dd, double theta_c, double q) {
+  double u = 1 + 2 * k_pi * k_pi * theta_c * theta_c * dd * dd * q * q;
+  return -(k_pi * k_pi * l * l * dd * dd * q * q * q * q / (2 * u));
+}
+
+double or_envelope_a(double dd, double theta_c, double q) {
+  double u = 1 + 2 * k_pi * k_pi * theta_c * theta_c * dd * dd * q * q;
+  return 1 / sqrt(u);
+}
+
+/* ctf.h:250-258 (A * exp(complex(log Es + log Et, -(chi - phi)))) over the
+ * grid of ctf.h:466-478. */
+void or_ctf_grid(double l, double df, double Cs, double Ca, double Pa,
+                 double dd, double theta_c, double phi, size_t w, size_t h,
+                 double ps, double *out) {
+  double inv_ps = 1.0 / ps;
+  size_t i, j;
+  for (j = 0; j < h; ++j) {
+    for (i = 0; i < w; ++i) {
+      double r, th, chi, les, let, a;
+      double complex e;
+      or_r_and_theta(i, j, w, h, inv_ps, &r, &th);
+      chi = or_chi(l, df, Cs, Ca, Pa, r, th);
+      let = or_log_et(l, dd, theta_c, r);
+      les = or_log_es(l, df, Cs, Ca, Pa, dd, theta_c, r, th);
+      a = or_envelope_a(dd, theta_c, r);
+      e = a * cexp(CMPLX(les + let, -(chi - phi)));
+      *out++ = creal(e);
+      *out++ = cimag(e);
+    }
+  }
+}
+
 /* ---- correction / filter exports ----------------------------------------- */
 
 void or_correct_f32(const float *image, const float *ctf, float *rec,

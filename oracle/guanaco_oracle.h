@@ -40,6 +40,19 @@ void or_ctf_simple_grid(double l, double df, double Cs, double Ca, double Pa,
                         double phi, size_t w, size_t h, double ps,
                         double *out);
 
+/* ctf.h:188-223: log spatial / temporal incoherence envelopes and the
+ * envelope scale A (De Graef eq. 10.53). */
+double or_log_es(double l, double df, double Cs, double Ca, double Pa,
+                 double dd, double theta_c, double q, double theta);
+double or_log_et(double l, double dd, double theta_c, double q);
+double or_envelope_a(double dd, double theta_c, double q);
+
+/* ctf.h:250-258 + 466-478: A exp(log Es + log Et) exp(-i(chi - phi)) over a
+ * w x h grid, interleaved (re, im) doubles. */
+void or_ctf_grid(double l, double df, double Cs, double Ca, double Pa,
+                 double dd, double theta_c, double phi, size_t w, size_t h,
+                 double ps, double *out);
+
 /* ---- FFT (stand-in for FFTW, fft.cpp:38-71) ------------------------------ */
 
 /* In-place complex FFT of n interleaved (re, im) values. sign = -1 forward,

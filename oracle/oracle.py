@@ -68,6 +68,14 @@ def lib():
                                      _f64p, _f64p]
         L.or_ctf_simple_grid.argtypes = [ctypes.c_double] * 6 + [
             _sz, _sz, ctypes.c_double, _f64p]
+        L.or_ctf_grid.argtypes = [ctypes.c_double] * 8 + [
+            _sz, _sz, ctypes.c_double, _f64p]
+        L.or_log_es.restype = ctypes.c_double
+        L.or_log_es.argtypes = [ctypes.c_double] * 9
+        L.or_log_et.restype = ctypes.c_double
+        L.or_log_et.argtypes = [ctypes.c_double] * 4
+        L.or_envelope_a.restype = ctypes.c_double
+        L.or_envelope_a.argtypes = [ctypes.c_double] * 3
         L.or_fft_f32.argtypes = [_sz, _f32p, ctypes.c_int]
         L.or_fft_f64.argtypes = [_sz, _f64p, ctypes.c_int]
         L.or_fft2_f32.argtypes = [_sz, _sz, _f32p, ctypes.c_int]
@@ -105,6 +113,10 @@ def ref_lib():
         R.ref_ctf_simple_grid.argtypes = [ctypes.c_double] * 6 + [
             _sz, _sz, ctypes.c_double, _f64p]
         R.ref_phase_flip.argtypes = [_f32p, _f32p, _f32p, _sz]
+        R.ref_ctf_grid.argtypes = [ctypes.c_double] * 8 + [
+            _sz, _sz, ctypes.c_double, _f64p]
+        R.ref_defocus_spread.restype = ctypes.c_double
+        R.ref_defocus_spread.argtypes = [ctypes.c_double] * 4
         _REF = R
     return _REF
 
@@ -143,6 +155,24 @@ def ctf_simple_grid(l, df=0.0, Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0, w=1, h=1,
     out = numpy.empty((h, w, 2), dtype=numpy.float64)
     lib().or_ctf_simple_grid(l, df, Cs, Ca, Pa, phi, w, h, float(ps), _p64(out))
     return out[..., 0] + 1j * out[..., 1]
+
+
+def ctf_grid(l, df=0.0, Cs=0.0, Ca=0.0, Pa=0.0, dd=0.0, theta_c=0.0,
+             phi=0.0, w=1, h=1, ps=1.0):
+    """complex128 (h, w) grid of the full CTF with envelopes (ctf.h:250-258,
+    466-478)."""
+    out = numpy.empty((h, w, 2), dtype=numpy.float64)
+    lib().or_ctf_grid(l, df, Cs, Ca, Pa, dd, theta_c, phi, w, h, float(ps),
+                      _p64(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def log_es(l, df, Cs, Ca, Pa, dd, theta_c, q, theta):
+    return lib().or_log_es(l, df, Cs, Ca, Pa, dd, theta_c, q, theta)
+
+
+def log_et(l, dd, theta_c, q):
+    return lib().or_log_et(l, dd, theta_c, q)
 
 
 # ---- FFT -----------------------------------------------------------------

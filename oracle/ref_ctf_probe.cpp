@@ -50,6 +50,28 @@ void ref_ctf_simple_grid(double l, double df, double Cs, double Ca, double Pa,
     c, reinterpret_cast<std::complex<double> *>(out), w, h, ps);
 }
 
+// get_ctf_n(c, ctf, w, h, ps) (the full CTF with envelopes) as called by the
+// pybind wrapper get_ctf_wh<double> (ext/ctf.h:131-139).
+void ref_ctf_grid(double l, double df, double Cs, double Ca, double Pa,
+                  double dd, double theta_c, double phi, size_t w, size_t h,
+                  double ps, double *out) {
+  guanaco::CTF c;
+  c.l = l;
+  c.df = df;
+  c.Cs = Cs;
+  c.Ca = Ca;
+  c.Pa = Pa;
+  c.dd = dd;
+  c.theta_c = theta_c;
+  c.phi = phi;
+  guanaco::get_ctf_n<double>(
+    c, reinterpret_cast<std::complex<double> *>(out), w, h, ps);
+}
+
+double ref_defocus_spread(double Cc, double dEE, double dII, double dVV) {
+  return guanaco::get_defocus_spread<double>(Cc, dEE, dII, dVV);
+}
+
 // phase_flip(x, h) from correct.h:30-39 on complex64 values.
 void ref_phase_flip(const float *x, const float *h, float *out, size_t n) {
   auto xc = reinterpret_cast<const std::complex<float> *>(x);
