@@ -68,6 +68,7 @@ class VolumeDesc(ctypes.Structure):
         ("Ca", ctypes.c_double), ("Pa", ctypes.c_double),
         ("phi", ctypes.c_double),
         ("tilt_chunk", _sz), ("gpu_index", _int),
+        ("centres", _f32p),
     ]
 
 
@@ -397,6 +398,11 @@ def volume_desc(num_pixels, num_rows, angles, depth, width=None,
         defocus, dtype=numpy.float64)
     width = num_pixels if width is None else width
     centre = num_pixels / 2.0 if centre is None else centre
+    centres = None
+    if numpy.size(centre) > 1:   # one centre per row
+        centres = numpy.ascontiguousarray(centre, dtype=numpy.float32)
+        assert centres.size == num_rows, "one centre per row"
+        centre = centres[0]
     desc = VolumeDesc(
         num_pixels, num_rows, T, num_defocus, depth, width,
         float(pixel_size), float(centre), float(min_defocus),
@@ -405,8 +411,8 @@ def volume_desc(num_pixels, num_rows, angles, depth, width=None,
         STORAGE[storage] if isinstance(storage, str) else storage,
         input_layout, None if defocus is None else f64p(defocus),
         float(wavelength), float(Cs), float(Ca), float(Pa), float(phi),
-        tilt_chunk, gpu_index)
-    return desc, (angles, weights, defocus)
+        tilt_chunk, gpu_index, None if centres is None else f32p(centres))
+    return desc, (angles, weights, defocus, centres)
 
 
 class VolumePlan:
@@ -560,7 +566,7 @@ def plan_bytes(**kw):
         STORAGE[storage] if isinstance(storage, str) else storage,
         kw.get("input_layout", INPUT_PROJECTIONS),
         None if defocus is None else f64p(defocus),
-        0.0, 0.0, 0.0, 0.0, 0.0, kw.get("tilt_chunk", 0), -1)
+        0.0, 0.0, 0.0, 0.0, 0.0, kw.get("tilt_chunk", 0), -1, None)
     n = _sz()
     call("gx_volume_plan_bytes", ctypes.byref(d), ctypes.byref(n))
     return n.value

@@ -177,10 +177,10 @@ py::tuple q_and_theta(std::size_t w, std::size_t h, double ps) {
 guanaco::VolumeRequest make_request(
     py::ssize_t num_pixels, py::ssize_t num_rows, py::ssize_t num_defocus,
     py::ssize_t depth, py::ssize_t width, const py::array_t<float> &angles,
-    int layout, float centre, float pixel_size, float min_defocus,
-    float max_defocus, const py::object &weights, const std::string &storage,
-    const py::object &defocus, double wavelength, double Cs, double Ca,
-    double Pa, double phi, int gpu_index) {
+    int layout, const py::object &centre, float pixel_size,
+    float min_defocus, float max_defocus, const py::object &weights,
+    const std::string &storage, const py::object &defocus, double wavelength,
+    double Cs, double Ca, double Pa, double phi, int gpu_index) {
   guanaco::VolumeRequest r;
   r.num_pixels = num_pixels;
   r.num_rows = num_rows;
@@ -189,7 +189,16 @@ guanaco::VolumeRequest make_request(
   r.depth = depth;
   r.width = width;
   r.pixel_size = pixel_size;
-  r.centre = centre;
+  // one centre, or one per row (reconstruct.py:32-52)
+  auto c = py::array_t<float, py::array::c_style | py::array::forcecast>(
+      centre);
+  if (c.size() == 1) {
+    r.centre = c.data()[0];
+  } else {
+    GUANACO_ASSERT(c.size() == num_rows);
+    r.centre = c.data()[0];
+    r.centres.assign(c.data(), c.data() + c.size());
+  }
   r.min_defocus = min_defocus;
   r.max_defocus = max_defocus;
   r.angles.assign(angles.data(), angles.data() + angles.size());
@@ -227,7 +236,7 @@ guanaco::VolumeRequest make_request(
 void reconstruct_volume(
     const py::array_t<float, py::array::c_style> &input,
     py::array_t<float, py::array::c_style> &volume,
-    const py::array_t<float> &angles, int layout, float centre,
+    const py::array_t<float> &angles, int layout, const py::object &centre,
     float pixel_size, float min_defocus, float max_defocus,
     const py::object &weights, const std::string &storage,
     const py::object &defocus, double wavelength, double Cs, double Ca,

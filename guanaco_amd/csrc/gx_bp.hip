@@ -41,6 +41,9 @@ __device__ __forceinline__ int2 kload(const int2 *p, size_t i) {
   kint *q = (kint *)p + 2 * i;
   return make_int2(q[0], q[1]);
 }
+__device__ __forceinline__ float kload(const float *p, size_t i) {
+  return ((kfloat *)p)[i];
+}
 
 template <int YB, int STORE>
 struct Taps;
@@ -366,6 +369,83 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
   }
 }
 
+// Per-row rotation centres (one centre per slice, mp.py:144-154): the
+// direct kernel with each row's own offset, so each row has its own taps.
+// The slab depends on h only, which no centre changes. kF32 keeps the
+// reference's order: p = (x c - z s) + offset_y, then p - 1/2.
+template <int YB, int STORE, bool MULTI>
+__global__ __launch_bounds__(kTX *kTZ) void k_bp_rows(
+    const typename StoreT<STORE>::T *__restrict__ slab,
+    const float4 *__restrict__ coef, const float *__restrict__ row_off,
+    int Y_tab, float dscale, float doffset, int ndf, int N, int T, int NYB,
+    int W, int Z, int Y, int yb0, int y_base, float *__restrict__ vol,
+    int nbx, int nbz, unsigned nblocks) {
+  constexpr bool kExact = STORE == kF32;
+  using ST = StoreT<STORE>;
+  const unsigned lb = xcd_remap(blockIdx.x, nblocks);
+  const int bz = (int)(lb % (unsigned)nbz);
+  const int bx = (int)((lb / (unsigned)nbz) % (unsigned)nbx);
+  const int byb = yb0 + (int)(lb / ((unsigned)nbz * (unsigned)nbx));
+  const int ix = bx * kTX + (int)(threadIdx.x % kTX);
+  const int iz = bz * kTZ + (int)(threadIdx.x / kTX);
+  const bool active = ix < W && iz < Z;
+  const float x = voxel_coord(ix, W);
+  const float z = voxel_coord(iz, Z);
+  // rows of this block that exist (the same for the whole workgroup)
+  const int nr = min(YB, Y - byb * YB);
+
+  float acc[YB];
+#pragma unroll
+  for (int r = 0; r < YB; ++r) acc[r] = 0.0f;
+
+  const size_t row_len = (size_t)N * YB;
+  const size_t kstride = (size_t)T * NYB * row_len;
+  const typename ST::T *yblock = slab + (size_t)byb * row_len;
+
+  for (int t = 0; t < T; ++t) {
+    const float4 cf = kload(coef, t);
+    int k = 0;
+    if constexpr (MULTI) k = slab_of(x, z, cf, dscale, doffset, ndf);
+    const typename ST::T *base =
+        yblock + (size_t)t * NYB * row_len + (size_t)k * kstride;
+    const float xc = x * cf.x - z * cf.y;  // kF32: the reference's first part
+    const size_t o = (size_t)t * Y_tab + (size_t)byb * YB;
+#pragma unroll
+    for (int r = 0; r < YB; ++r) {
+      if (r >= nr) break;
+      const float off = kload(row_off, o + r);
+      float p;
+      if constexpr (kExact) {
+        p = xc + off;
+        p = p - 0.5f;
+      } else {
+        p = fmaf(x, cf.x, fmaf(-z, cf.y, off));
+      }
+      const float fi = floorf(p);
+      const float tt = p - fi;
+      const int i0 = (int)fi;
+      if (active && (unsigned)i0 < (unsigned)(N - 1)) {
+        const float v0 = ST::unpack(base[(size_t)i0 * YB + r]);
+        const float v1 = ST::unpack(base[(size_t)(i0 + 1) * YB + r]);
+        if constexpr (kExact) {
+          acc[r] = acc[r] + cf.w * ((1.0f - tt) * v0 + tt * v1);
+        } else {
+          const float w1 = cf.w * tt;
+          acc[r] = fmaf(cf.w - w1, v0, acc[r]);
+          acc[r] = fmaf(w1, v1, acc[r]);
+        }
+      }
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int r = 0; r < YB; ++r) {
+      const int y = byb * YB + r;
+      if (y < Y) vol[((size_t)(y - y_base) * Z + iz) * W + ix] = acc[r];
+    }
+  }
+}
+
 template <int YB, int STORE, bool MULTI, int KS>
 __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
     const typename StoreT<STORE>::T *__restrict__ slab,
@@ -532,7 +612,44 @@ static hipError_t launch_bp_t(const BpArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int YB, int STORE>
+static hipError_t launch_bp_rows_t(const BpArgs &a, hipStream_t s) {
+  using ST = typename StoreT<STORE>::T;
+  const int nbx = (a.W + kTX - 1) / kTX;
+  const int nbz = (a.Z + kTZ - 1) / kTZ;
+  if (a.yb1 <= a.yb0) return hipSuccess;
+  const unsigned nb =
+      (unsigned)nbx * (unsigned)nbz * (unsigned)(a.yb1 - a.yb0);
+  if (a.ndf > 1) {
+    hipLaunchKernelGGL((k_bp_rows<YB, STORE, true>), dim3(nb),
+                       dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef,
+                       a.row_off, a.Y_tab, a.dscale, a.doffset, a.ndf, a.N,
+                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.vol,
+                       nbx, nbz, nb);
+  } else {
+    hipLaunchKernelGGL((k_bp_rows<YB, STORE, false>), dim3(nb),
+                       dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef,
+                       a.row_off, a.Y_tab, a.dscale, a.doffset, a.ndf, a.N,
+                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.vol,
+                       nbx, nbz, nb);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_bp(const BpArgs &a, int YB, int store, hipStream_t s) {
+  if (a.row_off != nullptr) {
+    if (store == kF32) {
+      if (YB == 8) return launch_bp_rows_t<8, kF32>(a, s);
+      if (YB == 16) return launch_bp_rows_t<16, kF32>(a, s);
+    } else if (store == kF16) {
+      if (YB == 16) return launch_bp_rows_t<16, kF16>(a, s);
+      if (YB == 32) return launch_bp_rows_t<32, kF16>(a, s);
+    } else if (store == kBF16) {
+      if (YB == 16) return launch_bp_rows_t<16, kBF16>(a, s);
+      if (YB == 32) return launch_bp_rows_t<32, kBF16>(a, s);
+    }
+    return hipErrorInvalidValue;
+  }
   if (a.staged) {
     if (store == kF32) {
       if (YB == 8) return launch_bp_lds_t<8, kF32>(a, s);

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -168,6 +169,73 @@ std::vector<float> ramp_table(size_t N, size_t L) {
 size_t filter_length(size_t N) {
   size_t L = next_pow2(2 * N - 1);
   return L < 8 ? 8 : L;
+}
+
+// The power-of-two correction kernels take 8 <= X <= 4096 and 4 <= Y <=
+// 4096; other sizes use Bluestein transforms of length filter_length(n).
+bool rows_pow2(size_t X) { return is_pow2(X) && X >= 8 && X <= 4096; }
+bool cols_pow2(size_t Y) { return is_pow2(Y) && Y >= 4 && Y <= 4096; }
+
+// In-place radix-2 FFT (sign -1) of n = 2^k complex doubles.
+void fft_f64(std::vector<std::complex<double>> &a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len) {
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+// Bluestein table entries of an X x Y correction (0 for power-of-two sizes).
+size_t bs_entries(size_t X, size_t Y) {
+  size_t n = 0;
+  if (!rows_pow2(X)) n += X + 2 * filter_length(X);
+  if (!cols_pow2(Y)) n += Y + 2 * filter_length(Y);
+  return n;
+}
+
+// Tables of an n-point Bluestein transform on M-point FFTs (M >= 2n - 1),
+// appended to out: chirp w_m = exp(-i pi m^2 / n) (n values, m^2 mod 2n in
+// integers), then FFT_M(b) / M and FFT_M(conj(b)) / M (M values each) with
+// b_m = conj(w_|m|) for |m| < n placed circularly.
+void bluestein_tables(size_t n, size_t M, std::vector<float2> &out) {
+  std::vector<std::complex<double>> w(n), b(M), bc(M);
+  for (size_t m = 0; m < n; ++m) {
+    const unsigned long long r =
+        ((unsigned long long)m * m) % (2ull * (unsigned long long)n);
+    const double a = -M_PI * (double)r / (double)n;
+    w[m] = std::complex<double>(std::cos(a), std::sin(a));
+  }
+  for (size_t m = 0; m < n; ++m) {
+    b[m] = std::conj(w[m]);
+    if (m) b[M - m] = std::conj(w[m]);
+  }
+  for (size_t m = 0; m < M; ++m) bc[m] = std::conj(b[m]);
+  fft_f64(b);
+  fft_f64(bc);
+  for (size_t m = 0; m < n; ++m)
+    out.push_back(make_float2((float)w[m].real(), (float)w[m].imag()));
+  for (const auto *v : {&b, &bc})
+    for (size_t m = 0; m < M; ++m)
+      out.push_back(make_float2((float)((*v)[m].real() / (double)M),
+                                (float)((*v)[m].imag() / (double)M)));
+}
+
+// BluesteinTable views of tables laid out by bluestein_tables at d.
+gx::BluesteinTable bluestein_view(const float2 *d, size_t n, size_t M) {
+  return gx::BluesteinTable{d, d + n, d + n + M, (int)n};
 }
 
 }  // namespace
@@ -544,11 +612,19 @@ extern "C" int gx_recon(const gx_recon_cfg *cfg, const float *sino,
 }
 
 static int check_corr_sizes(size_t X, size_t Y) {
-  if (!is_pow2(X) || X < 8 || X > 4096 || !is_pow2(Y) || Y < 4 || Y > 4096)
+  if (X < 1 || X > 4096 || Y < 1 || Y > 4096)
     return fail(GX_EUNSUPPORTED,
-                "CTF correction on gfx950 needs power-of-two sizes, "
-                "8 <= xsize <= 4096 and 4 <= ysize <= 4096 (got %zu x %zu)",
-                X, Y);
+                "CTF correction on gfx950 takes 1 <= xsize, ysize <= 4096 "
+                "(got %zu x %zu)", X, Y);
+  return GX_OK;
+}
+
+// The synthetic data generator uses the power-of-two correction kernels only.
+static int check_pow2_corr_sizes(size_t X, size_t Y) {
+  if (!rows_pow2(X) || !cols_pow2(Y))
+    return fail(GX_EUNSUPPORTED,
+                "synthetic CTF data needs power-of-two sizes, 8 <= xsize <= "
+                "4096 and 4 <= ysize <= 4096 (got %zu x %zu)", X, Y);
   return GX_OK;
 }
 
@@ -579,14 +655,44 @@ extern "C" int gx_corr(const float *image, const float *ctf, float *rec,
   GX_HIP(hipMemcpy(d_ctf.p, ctf, sizeof(float2) * npix * num_ctf,
                    hipMemcpyHostToDevice));
   hipStream_t s = nullptr;
+  const bool rp = rows_pow2(xsize), cp = cols_pow2(ysize);
+  const size_t MX = filter_length(xsize), MY = filter_length(ysize);
+  std::vector<float2> tab;
+  if (!rp) bluestein_tables(xsize, MX, tab);
+  const size_t ytab = tab.size();
+  if (!cp) bluestein_tables(ysize, MY, tab);
+  DevBuf d_tab;
+  if (!tab.empty()) {
+    GX_HIP(d_tab.alloc(sizeof(float2) * tab.size()));
+    GX_HIP(hipMemcpy(d_tab.p, tab.data(), sizeof(float2) * tab.size(),
+                     hipMemcpyHostToDevice));
+  }
+  const gx::BluesteinTable bx = bluestein_view(d_tab.as<float2>(), xsize, MX);
+  const gx::BluesteinTable by =
+      bluestein_view(d_tab.as<float2>() + ytab, ysize, MY);
   GX_HIP(launch_mask_from_ctf(d_mask.as<int8_t>(), d_ctf.as<float2>(), X, Y, XB,
                               ndf, s));
-  GX_HIP(launch_rows_r2c(d_img.as<float>(), npix, X, Y, XB, 1, d_F.as<float2>(),
-                         nullptr, 0, s));
-  GX_HIP(launch_cols(d_F.as<float2>(), d_mask.as<int8_t>(), d_G.as<float2>(), Y,
-                     XB, ndf, 1, s));
-  GX_HIP(launch_rows_c2r(d_G.as<float2>(), X, Y, XB, ndf, 1, d_out.as<float>(),
-                         s));
+  if (rp) {
+    GX_HIP(launch_rows_r2c(d_img.as<float>(), npix, X, Y, XB, 1,
+                           d_F.as<float2>(), nullptr, 0, s));
+  } else {
+    GX_HIP(launch_rows_r2c_bs(d_img.as<float>(), npix, X, Y, XB, 1,
+                              d_F.as<float2>(), nullptr, 0, (int)MX, bx, s));
+  }
+  if (cp) {
+    GX_HIP(launch_cols(d_F.as<float2>(), d_mask.as<int8_t>(),
+                       d_G.as<float2>(), Y, XB, ndf, 1, s));
+  } else {
+    GX_HIP(launch_cols_bs(d_F.as<float2>(), d_mask.as<int8_t>(),
+                          d_G.as<float2>(), Y, XB, ndf, 1, (int)MY, by, s));
+  }
+  if (rp) {
+    GX_HIP(launch_rows_c2r(d_G.as<float2>(), X, Y, XB, ndf, 1,
+                           d_out.as<float>(), s));
+  } else {
+    GX_HIP(launch_rows_c2r_bs(d_G.as<float2>(), X, Y, XB, ndf, 1,
+                              d_out.as<float>(), (int)MX, bx, s));
+  }
   GX_HIP(hipStreamSynchronize(s));
   GX_HIP(hipMemcpy(rec, d_out.p, sizeof(float) * npix * num_ctf,
                    hipMemcpyDeviceToHost));
@@ -609,6 +715,18 @@ struct gx_volume_plan {
   // table (bp_coefficients), rebuilt by every backprojection; win: the staged
   // kernel's per-(tile, tilt) windows. Tilt tables hold bp_tilt_pad(T) rows.
   DevBuf coef, coef_r, win, wscale, tmax, ramp, mask, defocus, F, G, slab;
+  // per-row rotation centres: row_off[t * Y + y], the tilt's offset for row
+  // y (reduced storage: offset - 1/2); empty when one centre serves all rows
+  DevBuf row_off;
+  bool per_row = false;
+  std::vector<float> h_row_off;
+  // Bluestein tables for sizes the power-of-two correction kernels do not
+  // take: rows (X points on MX) then columns (Y points on MY)
+  bool rows_bs = false, cols_bs = false;
+  int MX = 0, MY = 0;
+  size_t bs_y = 0;  // offset of the column tables
+  DevBuf bs;
+  std::vector<float2> h_bs;
   // host copies of the uploaded tables, alive until the plan is destroyed so
   // that stream-ordered uploads may read them
   std::vector<float> h_coef, h_wscale, h_ramp;
@@ -698,7 +816,7 @@ struct EventPair {
 namespace {
 
 constexpr size_t kPlanAlign = 256;
-constexpr int kPlanBuffers = 11;
+constexpr int kPlanBuffers = 13;
 
 struct PlanSizes {
   int YB, NYB, XB, L, chunk;
@@ -716,6 +834,10 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
   for (size_t t = 0; t < d->num_angles; ++t)
     if (!std::isfinite(d->angles[t]))
       return fail(GX_EINVAL, "angle %zu is not finite", t);
+  if (d->centres)
+    for (size_t y = 0; y < d->num_rows; ++y)
+      if (!std::isfinite(d->centres[y]))
+        return fail(GX_EINVAL, "centre of row %zu is not finite", y);
   GX_CHECK(d->storage >= GX_STORE_F32 && d->storage <= GX_STORE_BF16);
   GX_CHECK(d->input_layout >= GX_INPUT_PROJECTIONS &&
            d->input_layout <= GX_INPUT_SINOGRAMS);
@@ -735,7 +857,7 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
   ps->YB = d->storage == GX_STORE_F32 ? 16 : 32;
   ps->NYB = (int)((Y + ps->YB - 1) / ps->YB);
   ps->XB = (int)(X / 2 / CB + 1);
-  ps->L = (int)(ps->ctf ? 2 * X : filter_length(X));
+  ps->L = (int)(ps->ctf && rows_pow2(X) ? 2 * X : filter_length(X));
   const size_t es = d->storage == GX_STORE_F32 ? 4 : 2;
   ps->slab_bytes = ndf * T * (size_t)ps->NYB * ps->YB * X * es;
   const size_t nspec = (size_t)ps->XB * CB * Y;
@@ -760,7 +882,9 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
                     T * 2 * sizeof(float) +
                     bp_window_entries((int)d->width, (int)d->depth, (int)T) *
                         sizeof(int2) +
-                    sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf;
+                    sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf +
+                    (d->centres ? sizeof(float) * T * Y : 0) +
+                    (ps->ctf ? sizeof(float2) * bs_entries(X, Y) : 0);
   // every buffer starts on a kPlanAlign boundary inside a workspace
   ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
               ps->small_bytes + kPlanBuffers * kPlanAlign;
@@ -803,10 +927,24 @@ void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
   p->nrm_mul = ps.ctf ? (float)d->num_pixels * (float)d->num_rows : 1.0f;
   defocus_coefficients(d->num_defocus, d->pixel_size, d->min_defocus,
                        d->max_defocus, &p->dscale, &p->doffset);
-  p->bp_staged = bp_slots(d->num_defocus, p->dscale);
+  p->per_row = d->centres != nullptr;
+  p->bp_staged = p->per_row ? 0 : bp_slots(d->num_defocus, p->dscale);
   p->bp_kind = p->bp_staged ? GX_BP_STAGED : GX_BP_DIRECT;
   apply_tuned(p);
   const size_t T = d->num_angles;
+  if (p->per_row) {
+    // each row's offsets with the reference's arithmetic (tilt_geometry)
+    const size_t Y = d->num_rows;
+    std::vector<float> cf(4 * T);
+    p->h_row_off.assign(T * Y, 0.0f);
+    for (size_t y = 0; y < Y; ++y) {
+      tilt_geometry(T, d->angles, d->centres[y], cf.data());
+      for (size_t t = 0; t < T; ++t)
+        p->h_row_off[t * Y + y] = d->storage == GX_STORE_F32
+                                      ? cf[4 * t + 2]
+                                      : cf[4 * t + 2] - 0.5f;
+    }
+  }
   p->h_coef.assign(4 * (size_t)bp_tilt_pad((int)T), 0.0f);
   tilt_geometry(T, d->angles, d->centre, p->h_coef.data());
   p->h_wscale.assign(T, 1.0f);
@@ -814,6 +952,15 @@ void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
     for (size_t t = 0; t < T; ++t) p->h_wscale[t] = d->weights[t];
   p->h_ramp = ramp_table(d->num_pixels, (size_t)ps.L);
   if (ps.ctf) p->h_defocus.assign(d->defocus, d->defocus + d->num_defocus);
+  if (ps.ctf) {
+    p->rows_bs = !rows_pow2(d->num_pixels);
+    p->cols_bs = !cols_pow2(d->num_rows);
+    p->MX = (int)filter_length(d->num_pixels);
+    p->MY = (int)filter_length(d->num_rows);
+    if (p->rows_bs) bluestein_tables(d->num_pixels, p->MX, p->h_bs);
+    p->bs_y = p->h_bs.size();
+    if (p->cols_bs) bluestein_tables(d->num_rows, p->MY, p->h_bs);
+  }
   p->wavelength = d->wavelength;
   p->Cs = d->Cs;
   p->Ca = d->Ca;
@@ -836,7 +983,9 @@ void plan_buffers(gx_volume_plan *p, const PlanSizes &ps,
          {&p->defocus, ps.ctf ? sizeof(double) * p->ndf : 0},
          {&p->mask, ps.mask_bytes},
          {&p->F, ps.f_bytes},
-         {&p->G, ps.g_bytes}};
+         {&p->G, ps.g_bytes},
+         {&p->row_off, sizeof(float) * p->h_row_off.size()},
+         {&p->bs, sizeof(float2) * p->h_bs.size()}};
 }
 
 // Uploads the tables and initialises the device state on stream s.
@@ -858,6 +1007,14 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
   }
   GX_HIP(hipMemcpyAsync(p->wscale.p, p->h_wscale.data(), sizeof(float) * T,
                         hipMemcpyHostToDevice, s));
+  if (!p->h_bs.empty())
+    GX_HIP(hipMemcpyAsync(p->bs.p, p->h_bs.data(),
+                          sizeof(float2) * p->h_bs.size(),
+                          hipMemcpyHostToDevice, s));
+  if (p->per_row)
+    GX_HIP(hipMemcpyAsync(p->row_off.p, p->h_row_off.data(),
+                          sizeof(float) * p->h_row_off.size(),
+                          hipMemcpyHostToDevice, s));
   GX_HIP(hipMemcpyAsync(p->ramp.p, p->h_ramp.data(),
                         sizeof(float) * p->h_ramp.size(),
                         hipMemcpyHostToDevice, s));
@@ -955,17 +1112,40 @@ extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
   uint32_t *tmax = reduced ? p->tmax.as<uint32_t>() : nullptr;
   if (reduced) GX_HIP(launch_fill_u32(tmax, 0u, (size_t)p->T, s));
   const size_t X = p->X, Y = p->Y, T = p->T, ndf = p->ndf;
+  const gx::BluesteinTable bx =
+      bluestein_view(p->bs.as<float2>(), (size_t)p->X, (size_t)p->MX);
+  const gx::BluesteinTable by = bluestein_view(
+      p->bs.as<float2>() + p->bs_y, (size_t)p->Y, (size_t)p->MY);
   if (p->ctf) {
     for (int t0 = 0; t0 < p->T; t0 += p->chunk) {
       const int tc = p->T - t0 < p->chunk ? p->T - t0 : p->chunk;
-      GX_HIP(launch_rows_r2c(d_in + (size_t)t0 * Y * X, Y * X, p->X, p->Y,
-                             p->XB, tc, p->F.as<float2>(), tmax, t0, s));
-      GX_HIP(launch_cols(p->F.as<float2>(), p->mask.as<int8_t>(),
-                         p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s));
-      GX_HIP(launch_rows_c2r_filter(
-          p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
-          p->wscale.as<float>(), tmax, p->nrm_mul,
-          p->storage, p->slab.p, p->NYB, p->YB, s));
+      const float *img = d_in + (size_t)t0 * Y * X;
+      if (p->rows_bs) {
+        GX_HIP(launch_rows_r2c_bs(img, Y * X, p->X, p->Y, p->XB, tc,
+                                  p->F.as<float2>(), tmax, t0, p->MX, bx, s));
+      } else {
+        GX_HIP(launch_rows_r2c(img, Y * X, p->X, p->Y, p->XB, tc,
+                               p->F.as<float2>(), tmax, t0, s));
+      }
+      if (p->cols_bs) {
+        GX_HIP(launch_cols_bs(p->F.as<float2>(), p->mask.as<int8_t>(),
+                              p->G.as<float2>(), p->Y, p->XB, p->ndf, tc,
+                              p->MY, by, s));
+      } else {
+        GX_HIP(launch_cols(p->F.as<float2>(), p->mask.as<int8_t>(),
+                           p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s));
+      }
+      if (p->rows_bs) {
+        GX_HIP(launch_rows_c2r_filter_bs(
+            p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
+            p->ramp.as<float>(), p->wscale.as<float>(), tmax, p->nrm_mul,
+            p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s));
+      } else {
+        GX_HIP(launch_rows_c2r_filter(
+            p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
+            p->wscale.as<float>(), tmax, p->nrm_mul,
+            p->storage, p->slab.p, p->NYB, p->YB, s));
+      }
     }
   } else {
     size_t st, sk, sy;
@@ -1021,6 +1201,8 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   a.y_base = (int)y0;
   a.vol = d_vol;
   a.staged = p->bp_kind == GX_BP_STAGED ? p->bp_staged : 0;
+  a.row_off = p->per_row ? p->row_off.as<float>() : nullptr;
+  a.Y_tab = p->Y;
   GX_HIP(launch_bp(a, p->YB, p->storage, s));
   return GX_OK;
 }
@@ -1034,6 +1216,9 @@ extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
 extern "C" int gx_volume_plan_set_backprojector(gx_volume_plan *p, int kind) {
   GX_CHECK(p != nullptr);
   GX_CHECK(kind == GX_BP_DIRECT || kind == GX_BP_STAGED);
+  if (kind == GX_BP_STAGED && p->per_row)
+    return fail(GX_EUNSUPPORTED,
+                "per-row rotation centres need the direct backprojector");
   if (kind == GX_BP_STAGED && p->bp_staged == 0)
     return fail(GX_EUNSUPPORTED,
                 "a voxel tile spans more than 4 defocus slabs; the staged "
@@ -1249,7 +1434,7 @@ extern "C" int gx_synth_projections(const gx_synth_desc *d, float *d_out,
   GX_CHECK(d->num_ellipsoids <= 512);
   GX_CHECK(d->noise >= 0 && d->noise <= 2);
   if (d->ctf) {
-    int rc = check_corr_sizes(d->num_pixels, d->num_rows);
+    int rc = check_pow2_corr_sizes(d->num_pixels, d->num_rows);
     if (rc) return rc;
     GX_CHECK(d->pixel_size > 0);
   }

@@ -96,7 +96,7 @@ __global__ void k_ctf_mask(int8_t *__restrict__ mask, int X, int Y, int XB,
     const int m1 = flip_sign(sin(-(c1 - phi)));
     int m2 = m1;
     if (Ca != 0.0) {
-      const int mx = (X - kx) & (X - 1), my = (Y - ky) & (Y - 1);
+      const int mx = (X - kx) % X, my = (Y - ky) % Y;
       const double c2 = chi_at(mx, my, X, Y, inv_ps, l, df, Cs, Ca, Pa);
       m2 = flip_sign(sin(-(c2 - phi)));
     }
@@ -133,7 +133,7 @@ __global__ void k_mask_from_ctf(int8_t *__restrict__ mask,
   if (kx <= X / 2) {
     const float2 *c = ctf + (size_t)k * X * Y;
     const float a = c[(size_t)ky * X + kx].y;
-    const int mx = (X - kx) & (X - 1), my = (Y - ky) & (Y - 1);
+    const int mx = (X - kx) % X, my = (Y - ky) % Y;
     const float b = c[(size_t)my * X + mx].y;
     v = -((0.0f < a) - (a < 0.0f)) - ((0.0f < b) - (b < 0.0f));
   }
@@ -537,6 +537,84 @@ __global__ __launch_bounds__(2 * (L / default_e(L))) void k_filter_rows(
   }
 }
 
+// The volume path's filter: one workgroup filters RB = 4 * IT consecutive
+// rows y of one (tilt, slab), RB dividing the slab's row blocking, with the
+// outputs held in registers and written as whole 16-byte row runs (as
+// k_rows_c2r_filter does after the correction).
+template <int L, int STORE, int IT>
+__global__ __launch_bounds__(2 * (L / default_e(L))) void k_filter_rows_blk(
+    const float *__restrict__ src, size_t st, size_t sk, size_t sy, int N,
+    int Y, int T, const float *__restrict__ ramp,
+    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
+    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+    int YB, const float2 *__restrict__ tw) {
+  constexpr int EL = default_e(L);
+  constexpr int NT = L / EL;
+  using PT = typename RowPair<STORE>::T;
+  constexpr bool kWide = (IT * sizeof(PT)) % 16 == 0;
+  __shared__ float2 s[2][fft_lds_len(L, EL)];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int t = blockIdx.y, k = blockIdx.z;
+  const int r0 = blockIdx.x * (4 * IT) + p * (2 * IT);
+  const float *base = src + t * st + k * sk;
+  typename Fft<L, EL, -1, 1>::Twiddles twl;
+  twl.load(tid, tw);
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+  PT out[EL / 2][IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int ya = r0 + 2 * it, yb = ya + 1;
+    const float *a = base + (size_t)ya * sy;
+    const float *b = base + (size_t)yb * sy;
+    float2 u[EL];
+#pragma unroll
+    for (int m = 0; m < EL; ++m) {
+      const int n = tid + m * NT;
+      const float va = (ya < Y && n < N) ? a[n] : 0.0f;
+      const float vb = (yb < Y && n < N) ? b[n] : 0.0f;
+      u[m] = make_float2(va, vb);
+    }
+    Fft<L, EL, -1, 1>::run(u, s[p], tid, twl);
+#pragma unroll
+    for (int m = 0; m < EL; ++m) {
+      const int j = tid + m * NT;
+      const float r = ramp[j <= L / 2 ? j : L - j];
+      u[m] = make_float2(u[m].x * r, u[m].y * r);
+    }
+    Fft<L, EL, 1, 1>::run(u, s[p], tid, twl);
+#pragma unroll
+    for (int m = 0; m < EL / 2; ++m)
+      out[m][it] = RowPair<STORE>::pack(u[m].x * sc, u[m].y * sc);
+  }
+  using ST = typename StoreT<STORE>::T;
+  if (r0 >= Y) return;
+  const bool whole = r0 + 2 * IT <= Y;
+#pragma unroll
+  for (int m = 0; m < EL / 2; ++m) {
+    const int n = tid + m * NT;
+    if (n >= N) continue;
+    ST *dst = slab + slab_index(k, t, r0, n, T, NYB, N, YB);
+    if (whole && kWide) {
+      uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+      for (int q = 0; q < (int)(IT * sizeof(PT) / 16); ++q) {
+        uint4 w;
+        __builtin_memcpy(&w, reinterpret_cast<const char *>(out[m]) + 16 * q,
+                         16);
+        d4[q] = w;
+      }
+    } else if (whole) {
+      PT *dp = reinterpret_cast<PT *>(dst);
+#pragma unroll
+      for (int it = 0; it < IT; ++it) dp[it] = out[m][it];
+    } else {
+      const ST *src2 = reinterpret_cast<const ST *>(out[m]);
+      for (int r = 0; r < 2 * IT && r0 + r < Y; ++r) dst[r] = src2[r];
+    }
+  }
+}
+
 // max |row| per tilt over rows (k, t, y), for reduced-precision scaling.
 __global__ void k_absmax_rows(const float *__restrict__ src, size_t st,
                               size_t sk, size_t sy, int N, int Y, int T,
@@ -549,6 +627,234 @@ __global__ void k_absmax_rows(const float *__restrict__ src, size_t st,
   for (int n = threadIdx.x; n < N; n += blockDim.x) m = fmaxf(m, fabsf(a[n]));
   m = block_max<256>(m, red);
   if (threadIdx.x == 0) atomicMax(&tmax[t], __float_as_uint(m));
+}
+
+// ------------------------------------------ any size: Bluestein transforms -
+//
+// An n-point DFT for n not a power of two (or below the power-of-two kernels'
+// minimum) as a chirp-z transform on the M-point power-of-two FFTs, M >=
+// 2n - 1: with w_m = exp(-i pi m^2 / n), nk = (n^2 + k^2 - (n-k)^2) / 2 gives
+//   forward  X_k = w_k sum_j (x_j w_j) conj(w_(k-j)),
+//   inverse  x_j = conj(w_j) sum_k (X_k conj(w_k)) w_(j-k),
+// a circular convolution of length M computed as IFFT(FFT(u) FFT(b)) with
+// FFT(b) / M tabulated per plan (BluesteinTable). The chirp and the kernel
+// spectra are computed in FP64 on the host (m^2 mod 2n in integers) and
+// rounded once.
+// v[m] holds x[tid + m NT] for tid + m NT < n (anything beyond is ignored);
+// on return v[m] = X[tid + m NT] for indices < n and 0 beyond.
+template <int M, int E, int DIR, int LS>
+__device__ __forceinline__ void bluestein(
+    float2 (&v)[E], float2 *lds, int tid,
+    const typename Fft<M, E, -1, LS>::Twiddles &tw, const BluesteinTable &b) {
+  constexpr int NT = M / E;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int j = tid + m * NT;
+    if (j < b.n) {
+      const float2 c = b.chirp[j];
+      v[m] = cmul(v[m], DIR < 0 ? c : make_float2(c.x, -c.y));
+    } else {
+      v[m] = make_float2(0.0f, 0.0f);
+    }
+  }
+  Fft<M, E, -1, LS>::run(v, lds, tid, tw);
+  const float2 *B = DIR < 0 ? b.bf : b.bi;
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = cmul(v[m], B[tid + m * NT]);
+  Fft<M, E, 1, LS>::run(v, lds, tid, tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int j = tid + m * NT;
+    if (j < b.n) {
+      const float2 c = b.chirp[j];
+      v[m] = cmul(v[m], DIR < 0 ? c : make_float2(c.x, -c.y));
+    } else {
+      v[m] = make_float2(0.0f, 0.0f);
+    }
+  }
+}
+
+// Pass 1 for any X: rows (ya, yb) as one complex row, forward X-point DFT,
+// then the Hermitian split into the two half spectra (as k_rows_r2c).
+template <int M>
+__global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_r2c_bs(
+    const float *__restrict__ img, size_t tstride, int X, int Y, int XB,
+    float2 *__restrict__ F, uint32_t *__restrict__ tmax, int t0,
+    const float2 *__restrict__ tw, BluesteinTable bx) {
+  constexpr int E = default_e(M);
+  constexpr int NT = M / E;
+  __shared__ float2 s[2][fft_lds_len(M, E)];
+  __shared__ float red[16];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int t = blockIdx.y;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float *im = img + (size_t)t * tstride;
+  typename Fft<M, E, -1, 1>::Twiddles twm;
+  twm.load(tid, tw);
+  float2 v[E];
+  float amax = 0.0f;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    const float a = (ya < Y && n < X) ? im[(size_t)ya * X + n] : 0.0f;
+    const float b = (yb < Y && n < X) ? im[(size_t)yb * X + n] : 0.0f;
+    v[m] = make_float2(a, b);
+    amax = fmaxf(amax, fmaxf(fabsf(a), fabsf(b)));
+  }
+  bluestein<M, E, -1, 1>(v, s[p], tid, twm, bx);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    if (n < X) s[p][n] = v[m];
+  }
+  lds_barrier();
+  if (tmax != nullptr) {
+    amax = block_max<2 * NT>(amax, red);
+    if (threadIdx.x == 0) atomicMax(&tmax[t0 + t], __float_as_uint(amax));
+  }
+  const int ncol = XB * CB;
+  for (int e = threadIdx.x; e < 4 * ncol; e += 2 * NT) {
+    const int ci = e % CB, r = (e / CB) % 4, cb = e / (4 * CB);
+    const int kx = cb * CB + ci, y = blockIdx.x * 4 + r;
+    if (y >= Y) continue;
+    float2 val = make_float2(0.0f, 0.0f);
+    if (kx <= X / 2) {
+      const float2 z = s[r / 2][kx];
+      const float2 zm = s[r / 2][(X - kx) % X];
+      val = (r & 1) == 0 ? make_float2((z.x + zm.x) * 0.5f, (z.y - zm.y) * 0.5f)
+                         : make_float2((z.y + zm.y) * 0.5f, (zm.x - z.x) * 0.5f);
+    }
+    F[(((size_t)t * XB + cb) * Y + y) * CB + ci] = val;
+  }
+}
+
+// Pass 2 for any Y: CW of the CB columns of a block per workgroup (LDS
+// holds CW M-point transforms), forward Y-point DFT, then per defocus the
+// flip and the inverse Y-point DFT.
+template <int M, int CW>
+__global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
+    const float2 *__restrict__ F, const int8_t *__restrict__ mask,
+    float2 *__restrict__ G, int Y, int XB, int ndf,
+    const float2 *__restrict__ tw, BluesteinTable by) {
+  constexpr int E = default_e(M);
+  constexpr int NT = M / E;
+  static_assert(CB % CW == 0, "column groups split a block");
+  __shared__ float2 s[fft_lds_len(M, E) * CW];
+  const int c = threadIdx.x % CW, tid = threadIdx.x / CW;
+  const int cb = blockIdx.x / (CB / CW);
+  const int ci = (blockIdx.x % (CB / CW)) * CW + c;
+  const int t = blockIdx.y;
+  const float2 *src = F + ((size_t)t * XB + cb) * Y * CB;
+  typename Fft<M, E, -1, CW>::Twiddles twm;
+  twm.load(tid, tw);
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int y = tid + m * NT;
+    v[m] = y < Y ? src[(size_t)y * CB + ci] : make_float2(0.0f, 0.0f);
+  }
+  bluestein<M, E, -1, CW>(v, s + c, tid, twm, by);
+  for (int k = 0; k < ndf; ++k) {
+    const int8_t *mk = mask + ((size_t)k * XB + cb) * Y * CB;
+    float2 w[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int y = tid + m * NT;
+      const float f = y < Y ? mask_value(mk[(size_t)y * CB + ci]) : 0.0f;
+      w[m] = make_float2(v[m].x * f, v[m].y * f);
+    }
+    bluestein<M, E, 1, CW>(w, s + c, tid, twm, by);
+    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int y = tid + m * NT;
+      if (y < Y) dst[(size_t)y * CB + ci] = w[m];
+    }
+  }
+}
+
+// Pass 3 for any X, unfiltered (corr()): out[t][k][y][x].
+template <int M>
+__global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_c2r_bs(
+    const float2 *__restrict__ G, int X, int Y, int XB, int ndf,
+    float *__restrict__ out, const float2 *__restrict__ tw, BluesteinTable bx) {
+  constexpr int E = default_e(M);
+  constexpr int NT = M / E;
+  __shared__ float2 s[2][fft_lds_len(M, E)];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int k = blockIdx.y, tl = blockIdx.z;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+  typename Fft<M, E, -1, 1>::Twiddles twm;
+  twm.load(tid, tw);
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    v[m] = n < X ? load_pair_spectrum(g, X, Y, ya, yb, n)
+                 : make_float2(0.0f, 0.0f);
+  }
+  bluestein<M, E, 1, 1>(v, s[p], tid, twm, bx);
+  float *o = out + ((size_t)tl * ndf + k) * Y * X;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    if (n < X) {
+      if (ya < Y) o[(size_t)ya * X + n] = v[m].x;
+      if (yb < Y) o[(size_t)yb * X + n] = v[m].y;
+    }
+  }
+}
+
+// Pass 3 for any X, fused with the ramp filter: the corrected rows (X
+// samples, unnormalised inverse) are zero-padded to M >= 2X - 1 and filtered
+// with the truncated-kernel ramp table, as k_filter_rows does for real rows.
+template <int M, int STORE>
+__global__ __launch_bounds__(2 * (M / default_e(M))) void
+    k_rows_c2r_filter_bs(const float2 *__restrict__ G, int X, int Y, int XB,
+                         int ndf, int t0, int T,
+                         const float *__restrict__ ramp,
+                         const float *__restrict__ wscale,
+                         const uint32_t *__restrict__ tmax, float nrm_mul,
+                         typename StoreT<STORE>::T *__restrict__ slab,
+                         int NYB, int YB, const float2 *__restrict__ tw,
+                         BluesteinTable bx) {
+  constexpr int E = default_e(M);
+  constexpr int NT = M / E;
+  __shared__ float2 s[2][fft_lds_len(M, E)];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
+  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+  typename Fft<M, E, -1, 1>::Twiddles twm;
+  twm.load(tid, tw);
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    v[m] = n < X ? load_pair_spectrum(g, X, Y, ya, yb, n)
+                 : make_float2(0.0f, 0.0f);
+  }
+  bluestein<M, E, 1, 1>(v, s[p], tid, twm, bx);  // zero beyond X
+  Fft<M, E, -1, 1>::run(v, s[p], tid, twm);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int j = tid + m * NT;
+    const float r = ramp[j <= M / 2 ? j : M - j];
+    v[m] = make_float2(v[m].x * r, v[m].y * r);
+  }
+  Fft<M, E, 1, 1>::run(v, s[p], tid, twm);
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+  using ST = StoreT<STORE>;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    if (n < X) {
+      if (ya < Y) slab[slab_index(k, t, ya, n, T, NYB, X, YB)] = ST::pack(v[m].x * sc);
+      if (yb < Y) slab[slab_index(k, t, yb, n, T, NYB, X, YB)] = ST::pack(v[m].y * sc);
+    }
+  }
 }
 
 // ------------------------------------------------------------ launchers ---
@@ -749,6 +1055,33 @@ static hipError_t launch_filter_rows_t(const float *src, size_t st, size_t sk,
   return hipGetLastError();
 }
 
+template <int STORE, int IT>
+static hipError_t launch_filter_blk_it(const float *src, size_t st, size_t sk,
+                                       size_t sy, int N, int L, int Y, int T,
+                                       int ndf, const float *ramp,
+                                       const float *wscale,
+                                       const uint32_t *tmax, float nrm_mul,
+                                       void *slab, int NYB, int YB,
+                                       hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  dim3 grid((Y + 4 * IT - 1) / (4 * IT), T, ndf);
+  switch (L) {
+#define C_(n)                                                                  \
+  case n:                                                                      \
+    hipLaunchKernelGGL((k_filter_rows_blk<n, STORE, IT>), grid,                \
+                       dim3(2 * (n / default_e(n))), 0, s, src, st, sk, sy, N, \
+                       Y, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
+                       tw);                                                    \
+    break;
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+    C_(4096)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int L, int Y, int T, int ndf,
                               const float *ramp, const float *wscale,
@@ -757,6 +1090,23 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
+  // row blocks of whole 16-byte runs where the slab layout allows them
+  // (the 8192-point transforms keep the four-row form)
+  const int rb = c2r_filter_rows(L / 2, store, YB);
+  if (L <= 4096 && YB >= 8 && YB % rb == 0) {
+#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+    if (store == kF32) {
+      if (rb == 8) return launch_filter_blk_it<kF32, 2>(A_);
+      if (rb == 16) return launch_filter_blk_it<kF32, 4>(A_);
+    } else if (store == kF16) {
+      if (rb == 16) return launch_filter_blk_it<kF16, 4>(A_);
+      if (rb == 32) return launch_filter_blk_it<kF16, 8>(A_);
+    } else if (store == kBF16) {
+      if (rb == 16) return launch_filter_blk_it<kBF16, 4>(A_);
+      if (rb == 32) return launch_filter_blk_it<kBF16, 8>(A_);
+    }
+#undef A_
+  }
   switch (store) {
     case kF32:
       return launch_filter_rows_t<kF32>(src, st, sk, sy, N, L, Y, T, ndf,
@@ -781,6 +1131,135 @@ hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
   hipLaunchKernelGGL(k_absmax_rows, dim3((unsigned)nrows), dim3(256), 0, s,
                      src, st, sk, sy, N, Y, T, ndf, tmax);
   return hipGetLastError();
+}
+
+}  // namespace gx
+
+namespace gx {
+
+#define GX_BS_CASES(M) \
+  M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096) M(8192)
+
+hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
+                              int XB, int ntilts, float2 *F, uint32_t *tmax,
+                              int t0, int M, const BluesteinTable &bx,
+                              hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  dim3 grid((Y + 3) / 4, ntilts);
+  switch (M) {
+#define C_(n)                                                                \
+  case n:                                                                    \
+    hipLaunchKernelGGL(k_rows_r2c_bs<n>, grid, dim3(2 * (n / default_e(n))), \
+                       0, s, img, tstride, X, Y, XB, F, tmax, t0, tw, bx);   \
+    break;
+    GX_BS_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
+                          int Y, int XB, int ndf, int ntilts, int M,
+                          const BluesteinTable &by, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  // CB columns of a block per workgroup up to 4096-point transforms, half a
+  // block at 8192 points (LDS)
+  switch (M) {
+#define C_(n)                                                                 \
+  case n:                                                                     \
+    hipLaunchKernelGGL((k_cols_bs<n, CB>), dim3(XB, ntilts),                  \
+                       dim3(CB * (n / default_e(n))), 0, s, F, mask, G, Y,    \
+                       XB, ndf, tw, by);                                      \
+    break;
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+    C_(4096)
+#undef C_
+    case 8192:
+      hipLaunchKernelGGL((k_cols_bs<8192, CB / 2>), dim3(XB * 2, ntilts),
+                         dim3((CB / 2) * (8192 / default_e(8192))), 0, s, F,
+                         mask, G, Y, XB, ndf, tw, by);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_c2r_bs(const float2 *G, int X, int Y, int XB, int ndf,
+                              int ntilts, float *out, int M,
+                              const BluesteinTable &bx, hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+  dim3 grid((Y + 3) / 4, ndf, ntilts);
+  switch (M) {
+#define C_(n)                                                                \
+  case n:                                                                    \
+    hipLaunchKernelGGL(k_rows_c2r_bs<n>, grid, dim3(2 * (n / default_e(n))), \
+                       0, s, G, X, Y, XB, ndf, out, tw, bx);                 \
+    break;
+    GX_BS_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_c2r_filter_bs_t(const float2 *G, int X, int Y,
+                                         int XB, int ndf, int t0, int ntilts,
+                                         int T, const float *ramp,
+                                         const float *wscale,
+                                         const uint32_t *tmax, float nrm_mul,
+                                         void *slab, int NYB, int YB, int M,
+                                         const BluesteinTable &bx,
+                                         hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  dim3 grid((Y + 3) / 4, ndf, ntilts);
+  switch (M) {
+#define C_(n)                                                                  \
+  case n:                                                                      \
+    hipLaunchKernelGGL((k_rows_c2r_filter_bs<n, STORE>), grid,                 \
+                       dim3(2 * (n / default_e(n))), 0, s, G, X, Y, XB, ndf,   \
+                       t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
+                       tw, bx);                                                \
+    break;
+    GX_BS_CASES(C_)
+#undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
+                                     int ndf, int t0, int ntilts, int T,
+                                     const float *ramp, const float *wscale,
+                                     const uint32_t *tmax, float nrm_mul,
+                                     int store, void *slab, int NYB, int YB,
+                                     int M, const BluesteinTable &bx,
+                                     hipStream_t s) {
+  hipError_t err;
+  const float2 *tw = twiddles(&err);
+  if (err != hipSuccess) return err;
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, M, bx, s, tw
+  switch (store) {
+    case kF32:
+      return launch_c2r_filter_bs_t<kF32>(A_);
+    case kF16:
+      return launch_c2r_filter_bs_t<kF16>(A_);
+    case kBF16:
+      return launch_c2r_filter_bs_t<kBF16>(A_);
+  }
+#undef A_
+  return hipErrorInvalidValue;
 }
 
 }  // namespace gx

@@ -28,6 +28,10 @@ struct BpArgs {
   int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
   int y_base;                // vol holds rows [y_base, ...)
   float *vol;                // [y - y_base][z][x]
+  // per-row rotation centres: row_off[t * Y_tab + y] is the tilt's offset for
+  // row y (kF32: the reference's offset; reduced: offset - 1/2), or null
+  const float *row_off;
+  int Y_tab;
   int staged;                // 0: direct kernel; 2 or 4 (or 1 with one slab):
                              // LDS-staged kernel holding that many slabs
 };
@@ -37,6 +41,35 @@ struct BpArgs {
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_copy_u32(uint32_t *dst, const uint32_t *src, size_t n,
                            hipStream_t s);
+
+// Tables of an n-point Bluestein transform on M-point FFTs (device
+// pointers): chirp w_m = exp(-i pi m^2 / n) (m < n), and the spectra of the
+// convolution kernels for the forward and inverse transforms, divided by M.
+struct BluesteinTable {
+  const float2 *chirp;
+  const float2 *bf;
+  const float2 *bi;
+  int n;
+};
+// Correction passes for any X, Y (<= 4096): M = the Bluestein length,
+// a power of two >= 2n - 1 (and >= 8).
+hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
+                              int XB, int ntilts, float2 *F, uint32_t *tmax,
+                              int t0, int M, const BluesteinTable &bx,
+                              hipStream_t s);
+hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
+                          int Y, int XB, int ndf, int ntilts, int M,
+                          const BluesteinTable &by, hipStream_t s);
+hipError_t launch_rows_c2r_bs(const float2 *G, int X, int Y, int XB, int ndf,
+                              int ntilts, float *out, int M,
+                              const BluesteinTable &bx, hipStream_t s);
+hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
+                                     int ndf, int t0, int ntilts, int T,
+                                     const float *ramp, const float *wscale,
+                                     const uint32_t *tmax, float nrm_mul,
+                                     int store, void *slab, int NYB, int YB,
+                                     int M, const BluesteinTable &bx,
+                                     hipStream_t s);
 
 // CTF parameters in the reference's units (ctf.h:167-178): A and rad
 struct CtfParams {

@@ -34,7 +34,7 @@ size_t round_up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 // A request as the executor sees it: its own copy of the descriptor.
 struct Request {
   gx_volume_desc desc{};
-  std::vector<float> angles, weights;
+  std::vector<float> angles, weights, centres;
   std::vector<double> defocus;
   const float *input = nullptr;
   bool input_on_host = false;
@@ -268,6 +268,10 @@ extern "C" int gx_server_submit(gx_server *s, const gx_volume_desc *desc,
   if (desc->defocus) {
     r->defocus.assign(desc->defocus, desc->defocus + desc->num_defocus);
     r->desc.defocus = r->defocus.data();
+  }
+  if (desc->centres) {
+    r->centres.assign(desc->centres, desc->centres + desc->num_rows);
+    r->desc.centres = r->centres.data();
   }
   r->desc.gpu_index = -1;
   r->input = input;

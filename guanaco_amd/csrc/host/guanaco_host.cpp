@@ -219,7 +219,9 @@ void get_ctf_n_simple_imag(const CTF &c, double *ctf, std::size_t w,
 
 namespace {
 
-gx_volume_desc make_desc(const VolumeRequest &r, std::size_t rows) {
+// A descriptor of rows [y0, y0 + rows) of the request.
+gx_volume_desc make_desc(const VolumeRequest &r, std::size_t rows,
+                         std::size_t y0 = 0) {
   gx_volume_desc d{};
   d.num_pixels = r.num_pixels;
   d.num_rows = rows;
@@ -243,6 +245,7 @@ gx_volume_desc make_desc(const VolumeRequest &r, std::size_t rows) {
   d.phi = r.phi;
   d.tilt_chunk = 0;
   d.gpu_index = r.gpu_index;
+  d.centres = r.centres.empty() ? nullptr : r.centres.data() + y0;
   return d;
 }
 
@@ -256,6 +259,7 @@ void validate(const VolumeRequest &r) {
   GUANACO_ASSERT(r.angles.size() == r.num_angles);
   GUANACO_ASSERT(r.weights.empty() || r.weights.size() == r.num_angles);
   GUANACO_ASSERT(r.defocus.empty() || r.defocus.size() == r.num_defocus);
+  GUANACO_ASSERT(r.centres.empty() || r.centres.size() == r.num_rows);
 }
 
 // Copy detector rows [y0, y1) of the host input into a contiguous band in
@@ -329,6 +333,7 @@ void reconstruct_volume(const VolumeRequest &r, const float *input,
   // Row-local path: bands of detector rows, each a complete run.
   VolumeRequest one = r;
   one.num_rows = 1;
+  if (!one.centres.empty()) one.centres.resize(1);
   std::size_t per_row_plan = volume_plan_bytes(one);
   std::size_t band = budget / (per_row_plan + in_row + vol_row + 1);
   band = std::max<std::size_t>(1, std::min(Y, band));
@@ -336,7 +341,7 @@ void reconstruct_volume(const VolumeRequest &r, const float *input,
   std::vector<float> staging;
   for (std::size_t y0 = 0; y0 < Y; y0 += band) {
     const std::size_t y1 = std::min(Y, y0 + band), n = y1 - y0;
-    gx_volume_desc d = make_desc(r, n);
+    gx_volume_desc d = make_desc(r, n, y0);
     Plan plan(d);
     const float *src = input;
     if (n != Y) {

@@ -134,6 +134,7 @@ struct VolumeRequest {
   Layout layout = Layout::projections;
   Storage storage = Storage::f32;
   std::vector<double> defocus;  // empty: no CTF correction on device
+  std::vector<float> centres;   // per-row centres; empty: `centre` for all
   double wavelength = 0, Cs = 0, Ca = 0, Pa = 0, phi = 0;
   int gpu_index = -1;
 };

@@ -3,8 +3,9 @@
 For ``device="gpu"`` the whole volume is one device-resident call
 (``guanaco_ext.reconstruct_volume``), split into bands of detector rows only
 when it does not fit; every slice sees the same filtered data and geometry as
-the reference's per-slice ``recon`` calls. Per-slice dispatch (as the
-reference does it) remains for per-row rotation centres.
+the reference's per-slice ``recon`` calls, including a rotation centre per
+slice. The per-slice workers (as the reference dispatches them) serve the
+CPU device, which belongs to the reference and raises here.
 """
 import concurrent.futures as cf
 import multiprocessing as mp
@@ -48,27 +49,22 @@ def reconstruction_dispatcher(sinogram, reconstruction, centre, angles,
     """Reconstruct every slice of `sinogram` (Y, [ndf,] T, X) into
     `reconstruction` (Y, H, W) (mp.py:38-103).
 
-    With device "gpu" and one rotation centre for all rows, the volume is one
-    batched call on the original array `tomogram` (avoiding the
-    (T, ..., Y, X) -> (Y, ..., T, X) transpose copy).
+    With device "gpu" the volume is one batched call on the original array
+    `tomogram` (avoiding the (T, ..., Y, X) -> (Y, ..., T, X) transpose
+    copy), with one centre or one centre per row.
     """
     dev = ext.eDevice(device) if isinstance(device, (str, int)) else device
-    if dev == ext.eDevice.gpu and _uniform(centre):
+    if dev == ext.eDevice.gpu:
         gpu = -1 if not gpu_list else int(gpu_list[0])
         src = sinogram if tomogram is None else tomogram
         order = True if tomogram is None else sinogram_order
-        _volume_call(src, order, reconstruction, float(numpy.asarray(
-            centre).flat[0]), angles, weights, pixel_size, min_defocus,
-            max_defocus, transform, storage, gpu)
+        c = numpy.asarray(centre, dtype=numpy.float32)
+        _volume_call(src, order, reconstruction,
+                     float(c.flat[0]) if _uniform(c) else c, angles, weights,
+                     pixel_size, min_defocus, max_defocus, transform, storage,
+                     gpu)
         return
-    nslices = sinogram.shape[0]
-    if dev == ext.eDevice.gpu and ncore is None:
-        ncore = 1
-    if dev == ext.eDevice.gpu and gpu_list is not None:
-        ncore, slices = get_ncore_slices(nslices, len(gpu_list), nchunk)
-    else:
-        ncore, slices = get_ncore_slices(nslices, ncore, nchunk)
-        gpu_list = [None] * ncore
+    ncore, slices = get_ncore_slices(sinogram.shape[0], ncore, nchunk)
     args = (angles, weights, pixel_size, min_defocus, max_defocus, transform,
             device)
     if ncore == 1:
@@ -78,8 +74,8 @@ def reconstruction_dispatcher(sinogram, reconstruction, centre, angles,
     else:
         with cf.ThreadPoolExecutor(ncore) as e:
             futures = [e.submit(reconstruction_worker, sinogram[s],
-                                reconstruction[s], centre[s], *args, gpu)
-                       for gpu, s in zip(gpu_list, slices)]
+                                reconstruction[s], centre[s], *args, None)
+                       for s in slices]
             for f in futures:
                 f.result()
 

@@ -132,7 +132,6 @@ def reconstruct_projections(projections, angles, centre=None, pixel_size=1,
     assert projections.ndim == 3, "projections must be (T, Y, X)"
     T, Y, X = projections.shape
     c = get_centre(projections.shape, centre, False)
-    assert numpy.all(c == c[0]), "one rotation centre for all rows"
     weights = get_weights(angles, angular_weights).astype(numpy.float32)
     if transform == "minus":
         weights = -weights
@@ -157,7 +156,8 @@ def reconstruct_projections(projections, angles, centre=None, pixel_size=1,
         mn_rel, mx_rel = mn - defocus, mx - defocus
     ext.reconstruct_volume(projections, out,
                            numpy.asarray(angles, numpy.float32), 0,
-                           float(c[0]), pixel_size, mn_rel, mx_rel,
+                           float(c[0]) if numpy.all(c == c[0]) else c,
+                           pixel_size, mn_rel, mx_rel,
                            weights=weights, storage=storage,
                            gpu_index=gpu_index, **kw)
     return out

@@ -177,6 +177,11 @@ typedef struct gx_volume_desc {
   double phi;        /* rad */
   size_t tilt_chunk; /* tilts per correction launch; 0 = automatic */
   int gpu_index;     /* -1 = current device */
+  /* Per-row rotation centres (host, num_rows pixels), as reconstruct()
+   * accepts them (reconstruct.py:32-52, one per slice in mp.py:144-154), or
+   * NULL for `centre` on every row. With per-row centres the plan
+   * backprojects with the direct kernel, taps gathered per row. */
+  const float *centres;
 } gx_volume_desc;
 
 typedef struct gx_volume_plan gx_volume_plan;

@@ -111,13 +111,32 @@ def test_reconstruct_orders_weights_and_minus(capi):
     numpy.testing.assert_array_equal(sino, before)
 
 
-def test_reconstruct_per_row_centres_uses_slice_path(capi):
-    T, Y, X = 9, 6, 64
+def test_reconstruct_per_row_centres(capi):
+    T, Y, X = 9, 20, 64
     proj, angles = tilt_series(T, Y, X, 8, mean=0.0)
     centre = numpy.linspace(30.0, 34.0, Y).astype(numpy.float32)
     rec = G.reconstruct(proj, angles, centre=centre, device="gpu")
     expect = O.reconstruct(proj, angles, centre=centre)
     assert rel_l2(rec, expect) < 1e-5
+    e64 = O.reconstruct(proj, angles, centre=centre, precision="f64")
+    for storage, bound in (("f16", 1e-3), ("bf16", 4e-3)):
+        r = G.reconstruct(proj, angles, centre=centre, device="gpu",
+                          storage=storage)
+        assert rel_l2(r, e64) < bound, storage
+
+
+def test_fused_path_with_per_row_centres(capi):
+    T, Y, X, ndf = 11, 32, 128, 4
+    proj, angles = tilt_series(T, Y, X, 22)
+    centre = (64.0 + numpy.linspace(-3.0, 2.5, Y)).astype(numpy.float32)
+    kw = dict(pixel_size=1.0, energy=300, defocus=30000.0, num_defocus=ndf,
+              spherical_aberration=2.7, phase_shift=5.7392)
+    got = G.reconstruct_projections(proj, angles, centre=centre, depth=32,
+                                    **kw)
+    c, mn, mx = O.correct_projections(proj, centre, 1.0, 300, 30000.0, ndf,
+                                      None, 2.7, 0, 0, 5.7392)
+    expect = O.reconstruct(c, angles, centre, 1.0, mn, mx, depth=32)
+    assert rel_l2(got, expect) < 1e-5
 
 
 def test_reconstruct_into_memmap_output(capi, tmp_path):

@@ -83,13 +83,15 @@ def test_recon_rejects_invalid_config(kw, text):
 
 
 def test_corr_rejects_unsupported_sizes():
-    img = numpy.zeros((20, 30), numpy.float32)
-    ctf = numpy.zeros((20, 30), numpy.complex64)
-    out = numpy.zeros((20, 30), numpy.float32)
+    # any size up to 4096 x 4096 is taken (power-of-two kernels or Bluestein
+    # transforms); wider rows are refused before any device work
+    img = numpy.zeros((2, 5000), numpy.float32)
+    ctf = numpy.zeros((2, 5000), numpy.complex64)
+    out = numpy.zeros((2, 5000), numpy.float32)
     with pytest.raises(_capi.GxError) as e:
         _capi.call("gx_corr", _capi.f32p(img),
                    ctf.view(numpy.float32).ctypes.data_as(_capi._f32p),
-                   _capi.f32p(out), 30, 20, 1, -1)
+                   _capi.f32p(out), 5000, 2, 1, -1)
     assert e.value.code == _capi.GX_EUNSUPPORTED
 
 
@@ -109,8 +111,12 @@ def test_plan_footprint_is_host_only_arithmetic():
                            angles=angles, storage="f16", num_defocus=4,
                            defocus=numpy.linspace(1e4, 2e4, 4))
     assert ctf > 4 * f16 - (1 << 20)
+    # non-power-of-two corrections carry their Bluestein tables
+    odd = _capi.plan_bytes(num_pixels=1000, num_rows=1000, depth=256,
+                           angles=angles, defocus=[1e4], storage="f16")
+    assert odd > 121 * 1000 * 1000 * 2
     with pytest.raises(_capi.GxError):
-        _capi.plan_bytes(num_pixels=1000, num_rows=1024, depth=256,
+        _capi.plan_bytes(num_pixels=5000, num_rows=1024, depth=256,
                          angles=angles, defocus=[1e4])
 
 

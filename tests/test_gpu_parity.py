@@ -336,9 +336,14 @@ def slab_rows(plan, case, ys):
     (ndf, T, X) array per detector row in ys."""
     X, Y, T, ndf = case[:4]
     p, n, yb = plan.slabs()
-    a = numpy.empty(n // 4, numpy.float32)
-    C.call("gx_memcpy_dtoh", a.ctypes.data, p, n, None)
-    a = a.reshape(ndf, T, -1, X, yb)
+    # the buffer is sized for the largest row blocking; the layout in use
+    # holds ceil(Y / yb) blocks per (slab, tilt)
+    nyb = (Y + yb - 1) // yb
+    used = ndf * T * nyb * X * yb * 4
+    assert used <= n
+    a = numpy.empty(used // 4, numpy.float32)
+    C.call("gx_memcpy_dtoh", a.ctypes.data, p, used, None)
+    a = a.reshape(ndf, T, nyb, X, yb)
     return {y: numpy.ascontiguousarray(a[:, :, y // yb, :, y % yb])
             for y in ys}
 
@@ -448,3 +453,40 @@ def test_autotuner_candidates_agree_and_choice_is_remembered(
     p4.destroy()
     plan.destroy()
     C.tune_clear()
+
+
+# ------------------------------------------------- per-row rotation centres
+
+@pytest.mark.parametrize("case", [BP_KIND_CASES[0], BP_KIND_CASES[2],
+                                  BP_KIND_CASES[4]])
+def test_per_row_centre_bp_bitexact_vs_oracle(capi, case):
+    X, Y, T, ndf, Z, W, (a0, a1), (mn, mx), ps = case
+    angles = numpy.deg2rad(numpy.linspace(a0, a1, T)).astype(numpy.float32)
+    rng = numpy.random.default_rng(X + T + ndf + 7)
+    sino = rng.standard_normal((Y, ndf, T, X)).astype(numpy.float32)
+    centres = (X / 2.0 + rng.uniform(-4.0, 4.0, Y)).astype(numpy.float32)
+    plan = C.VolumePlan(X, Y, angles, Z, width=W, num_defocus=ndf,
+                        pixel_size=ps, centre=centres, min_defocus=mn,
+                        max_defocus=mx, storage="f32",
+                        input_layout=C.INPUT_SINOGRAMS)
+    assert plan.backprojector() == (C.BP_DIRECT, 0)
+    with pytest.raises(C.GxError, match="per-row rotation centres"):
+        plan.set_backprojector("staged")
+    d_in = C.DeviceArray.from_host(sino)
+    ys = [0, Y // 2, Y - 1]
+    for rows in (16, 8):
+        plan.set_tile(rows, "direct")
+        d_vol = C.DeviceArray((Y, Z, W))
+        C.call("gx_fill_u32", d_vol.ptr, 0x7fc00000, Y * Z * W, None)
+        plan.run(d_in, d_vol)
+        C.call("gx_device_synchronize")
+        vol = d_vol.to_host()
+        srows = slab_rows(plan, case, ys)
+        for y in ys:
+            s = srows[y] if ndf > 1 else srows[y][0]
+            expect = O.project(s, (Z, W), angles, float(centres[y]), ps, mn,
+                               mx)
+            numpy.testing.assert_array_equal(vol[y], expect,
+                                             err_msg="row %d, %d rows" % (
+                                                 y, rows))
+    plan.destroy()

@@ -131,3 +131,15 @@ def test_admission_refusal_and_cancellation(capi):
         for r in (r2, r3, r4):
             r.release()
         assert srv.stats()["used_bytes"] == 0
+
+
+def test_served_request_with_per_row_centres(capi):
+    proj, angles, Z, kw = case(64, 15, 2, 16, "f32", 12)
+    centres = (32.0 + numpy.linspace(-2.0, 2.0, proj.shape[1])).astype(
+        numpy.float32)
+    kw = dict(kw, centre=centres)
+    with Server(budget_bytes=1 << 28) as srv:
+        r = srv.submit(proj, angles, Z, **kw)
+        got = r.result()
+        r.release()
+    numpy.testing.assert_array_equal(got, plan_result(proj, angles, Z, kw))
