@@ -220,16 +220,27 @@ def host_path(w, angles, proj, storage, reps=3):
 
 # ---------------------------------------------------- CPU oracle legs -----
 
+def _oracle_defoci(w):
+    """Absolute defoci of the slabs and the relative range
+    (correct.py:44-93, 263-318)."""
+    centre = numpy.full(w["Y"], w["X"] / 2.0, numpy.float32)
+    mn, mx, st, n = _defocus_steps(w["df"], w["ndf"], w["X"], centre, w["ps"])
+    return [mn + d * st for d in range(n)], mn - w["df"], mx - w["df"]
+
+
 def _oracle_correct_rows(args):
-    """FP32 and FP64 oracle correction of one tilt; only sampled rows kept."""
+    """FP32 and FP64 oracle correction of one tilt for every defocus; only
+    the sampled rows kept: (ndf, nrows, X) each."""
     from oracle import oracle as O
     img, w, rows = args
     l = O.electron_wavelength(300000)
-    ctf = O.ctf_simple_grid(l, w["df"], 2.7e7, 0, 0, w["phi"], w["X"], w["Y"],
-                            w["ps"]).astype(numpy.complex64)
+    dfs, _, _ = _oracle_defoci(w)
+    ctf = numpy.stack([O.ctf_simple_grid(l, df, 2.7e7, 0, 0, w["phi"], w["X"],
+                                         w["Y"], w["ps"]) for df in dfs])
+    ctf = ctf.astype(numpy.complex64)
     c32 = O.corr(img, ctf, precision="f32")
     c64 = O.corr(img, ctf, precision="f64")
-    return c32[rows], c64[rows]
+    return c32[:, rows], c64[:, rows]
 
 
 def oracle_check(w, proj, angles, rows, vols, pool, dump=None):
@@ -237,21 +248,28 @@ def oracle_check(w, proj, angles, rows, vols, pool, dump=None):
     oracle chains (correct_projections + reconstruct)."""
     from oracle import oracle as O
     T = proj.shape[0]
+    mn = mx = 0.0
     if w["df"] is None:
-        s32 = numpy.ascontiguousarray(proj[:, rows, :])
+        s32 = numpy.ascontiguousarray(proj[:, None, rows, :])
         s64 = s32.astype(numpy.float64)
     else:
+        _, mn, mx = _oracle_defoci(w)
         res = pool.map(_oracle_correct_rows,
                        [(proj[t], w, rows) for t in range(T)])
-        s32 = numpy.stack([r[0] for r in res])  # (T, nrows, X)
+        s32 = numpy.stack([r[0] for r in res])  # (T, ndf, nrows, X)
         s64 = numpy.stack([r[1] for r in res])
     out = {}
-    ref32 = numpy.stack([O.recon(numpy.ascontiguousarray(s32[:, i, :]),
-                                 (w["Z"], w["X"]), angles, w["X"] / 2.0,
-                                 w["ps"]) for i in range(len(rows))])
-    ref64 = numpy.stack([O.recon(numpy.ascontiguousarray(s64[:, i, :]),
-                                 (w["Z"], w["X"]), angles, w["X"] / 2.0,
-                                 w["ps"], precision="f64")
+
+    def sino(s, i):  # (ndf, T, X) of sampled row i, or (T, X) for one slab
+        a = numpy.ascontiguousarray(numpy.swapaxes(s[:, :, i, :], 0, 1))
+        return a[0] if a.shape[0] == 1 else a
+
+    ref32 = numpy.stack([O.recon(sino(s32, i), (w["Z"], w["X"]), angles,
+                                 w["X"] / 2.0, w["ps"], mn, mx)
+                         for i in range(len(rows))])
+    ref64 = numpy.stack([O.recon(sino(s64, i), (w["Z"], w["X"]), angles,
+                                 w["X"] / 2.0, w["ps"], mn, mx,
+                                 precision="f64")
                          for i in range(len(rows))])
     if dump:
         os.makedirs(dump, exist_ok=True)
@@ -497,7 +515,7 @@ def main():
     if args.config == "c5":
         return main_serving(args)
     w = WORKLOADS[args.config]
-    check = not args.no_check and args.only is None and w["ndf"] == 1
+    check = not args.no_check and args.only is None
     # CPU workers for the oracle check are forked before anything touches
     # the GPU, so they inherit no device state
     pool = None

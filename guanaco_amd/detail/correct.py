@@ -15,7 +15,7 @@ import guanaco_amd.detail as ext
 from guanaco_amd import mrc
 
 __all__ = ["correct_projections", "correct_file", "get_defocus_steps",
-           "get_ctf", "get_ctf_array"]
+           "get_corrected_array", "get_ctf", "get_ctf_array"]
 
 
 def get_centre(shape, centre, sinogram_order=True):

@@ -17,7 +17,8 @@ from guanaco_amd.detail import mp
 from guanaco_amd.detail.correct import correct_projections, get_defocus_steps
 
 __all__ = ["reconstruct", "reconstruct_projections", "reconstruct_file",
-           "get_centre", "compute_angular_weights", "get_weights"]
+           "reconstruct_tomopy", "get_centre", "compute_angular_weights",
+           "get_weights"]
 
 
 def get_centre(shape, centre, sinogram_order=True):
@@ -260,9 +261,8 @@ def reconstruct_file(input_filename, output_filename,
         if corrected:
             method = "FBP_CTF"
         if method != "FBP_CTF":
-            raise NotImplementedError(
-                "method %s is the reference's tomopy path; this package "
-                "implements FBP_CTF" % method)
+            # the third-party algorithms: checked before any output is made
+            _tomopy()
         run_fused = fused and not corrected and defocus is not None
         if not corrected and defocus is not None and not run_fused:
             projections, min_defocus, max_defocus = correct_projections(
@@ -274,7 +274,12 @@ def reconstruct_file(input_filename, output_filename,
         with mrc.new_mmap(output_filename, output_shape, mrc_mode=2,
                           overwrite=True) as outfile:
             outfile.voxel_size = voxel_size
-            if run_fused:
+            if method != "FBP_CTF":
+                reconstruct_tomopy(projections, angles, outfile.data,
+                                   centre=centre, transform=transform,
+                                   device=device, ncore=ncore, method=method,
+                                   num_iter=num_iter)
+            elif run_fused:
                 reconstruct_projections(
                     projections, angles, centre=centre,
                     pixel_size=pixel_size, energy=energy, defocus=defocus,
@@ -293,3 +298,37 @@ def reconstruct_file(input_filename, output_filename,
                             angular_weights=angular_weights, device=device,
                             ncore=ncore, storage=storage, depth=depth)
     print("Time: %.2f seconds" % (time.time() - start_time))
+
+
+def _tomopy():
+    try:
+        import tomopy
+    except ImportError:
+        raise RuntimeError(
+            "The reconstruction algorithm selected requires tomopy and astra")
+    return tomopy
+
+
+def reconstruct_tomopy(projections, angles, reconstruction, centre=None,
+                       transform=None, device="cpu", ncore=None, method="FBP",
+                       num_iter=200, output_rot90=False):
+    """The reference's hand-over of methods other than FBP_CTF (SIRT, SART,
+    CGLS, EM, FBP) to tomopy with the astra toolbox
+    (reconstruct.py:405-451). tomopy is a third-party package, not part of
+    this one: without it the call raises RuntimeError, as the reference's
+    does. With it, the astra options follow the reference's: the "linear"
+    projector on the CPU, the "<method>_CUDA" algorithms otherwise."""
+    tomopy = _tomopy()
+    print("Using %s" % device)
+    if device == "cpu":
+        options = {"proj_type": "linear", "method": method}
+    else:
+        options = {"proj_type": "cuda", "method": "%s_CUDA" % method}
+        ncore = 1 if ncore is None else ncore
+    if method in ("SIRT", "SART", "CGLS"):
+        options["num_iter"] = num_iter
+    if transform == "minus":
+        projections = -projections
+    reconstruction[:] = tomopy.recon(projections, angles,
+                                     algorithm=tomopy.astra, options=options,
+                                     ncore=ncore)

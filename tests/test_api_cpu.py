@@ -14,8 +14,18 @@ def test_public_names_match_reference_surface():
     # src/guanaco/__init__.py:22-28, detail/__init__.py:21,
     # ext/guanaco_ext.cpp:32-37
     for name in ["reconstruct", "correct_projections", "recon", "corr",
-                 "eDevice", "CTF", "get_electron_wavelength"]:
+                 "eDevice", "CTF", "get_electron_wavelength",
+                 "reconstruct_file", "correct_file", "reconstruct_tomopy",
+                 "get_centre", "get_defocus_steps", "get_corrected_array",
+                 "get_ctf", "get_ctf_array", "compute_angular_weights",
+                 "get_weights", "get_defocus_spread", "get_q_and_theta",
+                 "q_to_Q", "Q_to_q", "df_to_D", "D_to_df"]:
         assert hasattr(G, name), name
+    import guanaco_amd.plot as P
+    assert callable(P.plot_ctf) and callable(P.generate_ctf)
+    with pytest.raises(RuntimeError, match="requires tomopy"):
+        G.reconstruct_tomopy(numpy.zeros((3, 2, 8), numpy.float32),
+                             numpy.zeros(3), numpy.zeros((2, 8, 8)))
     assert hasattr(G.detail, "recon") and hasattr(G.detail, "corr")
 
 

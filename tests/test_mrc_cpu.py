@@ -230,7 +230,7 @@ def test_file_drivers_refuse_cpu_and_other_methods(tmp_path):
                            pixel_size=1.0)
     with pytest.raises(RuntimeError, match="GPU device"):
         G.correct_file(str(p), str(out), defocus=1e4)
-    with pytest.raises(NotImplementedError, match="tomopy"):
+    with pytest.raises(RuntimeError, match="requires tomopy"):
         G.reconstruct_file(str(p), str(out), start_angle=-10, step_angle=10,
                            pixel_size=1.0, method="SIRT", device="gpu")
     with pytest.raises(ValueError, match="extended header"):
