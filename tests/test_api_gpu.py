@@ -147,3 +147,23 @@ def test_reconstruct_into_memmap_output(capi, tmp_path):
     G.reconstruct(proj, angles, reconstruction=out, device="gpu")
     expect = O.reconstruct(proj, angles)
     assert rel_l2(numpy.asarray(out), expect) < 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_fused_path_4096_point_rows(capi, storage):
+    # the widest rows: 4096-point corrections and 8192-point filter
+    # transforms (one row pair per slot per workgroup)
+    T, Y, X, ndf = 3, 8, 4096, 2
+    proj, angles = tilt_series(T, Y, X, 41)
+    kw = dict(pixel_size=1.0, energy=300, defocus=25000.0, num_defocus=ndf,
+              spherical_aberration=2.7, phase_shift=5.7392)
+    got = G.reconstruct_projections(proj, angles, depth=8, storage=storage,
+                                    **kw)
+    centre = O.get_centre(proj.shape, None, False)
+    prec = "f32" if storage == "f32" else "f64"
+    c, mn, mx = O.correct_projections(proj, centre, 1.0, 300, 25000.0, ndf,
+                                      None, 2.7, 0, 0, 5.7392,
+                                      precision=prec)
+    expect = O.reconstruct(c, angles, None, 1.0, mn, mx, depth=8,
+                           precision=prec)
+    assert rel_l2(got, expect) < (1e-5 if storage == "f32" else 1e-3)
