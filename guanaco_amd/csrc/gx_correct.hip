@@ -359,6 +359,7 @@ struct RowPair<kBF16> {
   }
 };
 
+// Rows of fewer than 256 points (k_rows_c2r_ramp below takes the others).
 // One workgroup filters RB = 4 * IT consecutive rows of one (tilt, defocus)
 // spectrum, RB dividing the slab's row blocking: two pair slots (L = 2X
 // threads-per-pair layout shared by the X-point inverse, end radix EL/2, and
@@ -447,6 +448,133 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
     } else {
       const ST *src = reinterpret_cast<const ST *>(out[m]);
       for (int r = 0; r < 2 * IT && r0 + r < Y; ++r) dst[r] = src[r];
+    }
+  }
+}
+
+// The same ramp-filtered inverse as k_rows_c2r_filter, as four X-point
+// transforms per row pair instead of one X-point and two 2X-point ones. With
+// z = IFFT_X(Z) (the row pair, unnormalised) zero-padded to L = 2X:
+//   FFT_L(z)[2i]     = X Z[i]
+//   FFT_L(z)[2i + 1] = FFT_X(z[n] c(n))[i],          c(n) = exp(-i pi n / X)
+//   IFFT_L(U)[n]     = IFFT_X(U[2i])[n] + conj(c(n)) IFFT_X(U[2i + 1])[n]
+// for n < X, so the filtered rows are IFFT_X(X r[2i] Z) + conj(c) IFFT_X(
+// r[2i + 1] FFT_X(c z)), r the ramp of the 2X grid. Every transform is X
+// points: a slot (one row pair) needs X / 16 threads and an X-point LDS
+// region, and at X <= 1024 a slot lies within one wave and synchronises
+// only itself. The P slots of a workgroup (RB = 2P rows dividing the slab's
+// row blocking) then meet once, to stage their outputs in LDS column by
+// column, so that the workgroup writes each column's RB rows as whole
+// 8- or 16-byte runs.
+template <int X, int STORE, int P>
+__global__ __launch_bounds__(P *(X / default_e(X))) void k_rows_c2r_ramp(
+    const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
+    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
+    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+    int YB, const float2 *__restrict__ tw) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  constexpr bool WAVE = NT <= 64;
+  constexpr int L = 2 * X;
+  constexpr int SL = fft_lds_len(X, E);
+  using PT = typename RowPair<STORE>::T;
+  using ST = typename StoreT<STORE>::T;
+  static_assert(E == 16, "sizes of 256 points and more");
+  __shared__ float2 s[P * SL];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
+  const int R0 = blockIdx.x * (2 * P);
+  const int ya = R0 + 2 * p, yb = ya + 1;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+  float2 *ls = s + p * SL;
+
+  typename Fft<X, E, 1, 1, WAVE>::Twiddles twx;  // both directions
+  twx.load(tid, tw);
+  // c(tid + m NT) = c(tid) exp(-i pi m / E); both factors from the W^j table
+  constexpr int kHalf = kTwN / L;  // W^(kHalf n) = c(n)
+  const float2 ct = tw[tid * kHalf];
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+  // filter.cpp:30-45: r[j] = |j| / X on the 2X grid, times the 1/L of the
+  // unnormalised inverse (powers of two: exact as the reference's product)
+  constexpr float kRamp = 1.0f / ((float)X * (float)L);
+
+  float2 z[E], e[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    z[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+  // even bins: X r[2n] Z[n] = (fold(2n) / L) Z[n]
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    const float r = (float)(2 * n <= X ? 2 * n : L - 2 * n) * (1.0f / L);
+    e[m] = make_float2(z[m].x * r, z[m].y * r);
+  }
+  Fft<X, E, 1, 1, WAVE>::run(e, ls, tid, twx);
+  // odd bins: the rows, twiddled by c(n), forward, times r[2n + 1]
+  Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+    z[m] = cmul(z[m], cmul(ct, tw[m * (kTwN / (2 * E))]));
+  Fft<X, E, -1, 1, WAVE>::run(z, ls, tid, twx);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int n = tid + m * NT;
+    const float r = (float)(2 * n + 1 <= X ? 2 * n + 1 : L - 2 * n - 1) * kRamp;
+    z[m] = make_float2(z[m].x * r, z[m].y * r);
+  }
+  Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
+  PT out[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const float2 c = cmul(ct, tw[m * (kTwN / (2 * E))]);
+    const float2 o = cmul(z[m], make_float2(c.x, -c.y));
+    out[m] = RowPair<STORE>::pack((e[m].x + o.x) * sc, (e[m].y + o.y) * sc);
+  }
+
+  if (P == 1 || R0 + 2 * P > Y) {  // one slot, or the last, partial block
+    if (ya >= Y) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      ST *dst = slab + slab_index(k, t, ya, tid + m * NT, T, NYB, X, YB);
+      const ST *v = reinterpret_cast<const ST *>(&out[m]);
+      dst[0] = v[0];
+      if (yb < Y) dst[1] = v[1];
+    }
+    return;
+  }
+  // Stage [column][slot] (stride P + 1 PT: odd in 4-byte words for 16-bit
+  // storage, so a wave's column writes fall in distinct banks), in NCH passes
+  // of XC columns when the slots' regions do not hold all X columns at once.
+  constexpr int NCH = sizeof(PT) / 4;
+  constexpr int XC = X / NCH;
+  constexpr int XS = P + 1;
+  static_assert((size_t)XC * XS * sizeof(PT) <= sizeof(s), "staging fits");
+  constexpr int QW = P * sizeof(PT) < 16 ? P : 16 / (int)sizeof(PT);
+  constexpr int NQ = P / QW;  // runs per column
+  using RunT = typename std::conditional<
+      QW * sizeof(PT) == 16, uint4,
+      typename std::conditional<QW * sizeof(PT) == 8, uint2,
+                                uint32_t>::type>::type;
+  static_assert(QW * sizeof(PT) == sizeof(RunT), "whole runs");
+  PT *st = reinterpret_cast<PT *>(s);
+  ST *base = slab + slab_index(k, t, R0, 0, T, NYB, X, YB);
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    lds_barrier();  // the slots' transforms, or the previous pass, are done
+#pragma unroll
+    for (int m = c * (E / NCH); m < (c + 1) * (E / NCH); ++m)
+      st[(tid + m * NT - c * XC) * XS + p] = out[m];
+    lds_barrier();
+    for (int q = threadIdx.x; q < XC * NQ; q += P * NT) {
+      const int x = q / NQ, j = q % NQ;
+      PT v[QW];
+#pragma unroll
+      for (int i = 0; i < QW; ++i) v[i] = st[x * XS + j * QW + i];
+      RunT w;
+      __builtin_memcpy(&w, v, sizeof(RunT));
+      *reinterpret_cast<RunT *>(base + (size_t)(c * XC + x) * YB +
+                                2 * QW * j) = w;
     }
   }
 }
@@ -942,20 +1070,63 @@ static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
                        ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
                        tw);                                                    \
     break;
-    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
+    C_(8) C_(16) C_(32) C_(64) C_(128)
 #undef C_
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Slots (row pairs) per workgroup of k_rows_c2r_ramp: RB = 2P rows dividing
+// the slab's row blocking. One-wave slots (X <= 1024) run 4 to a workgroup;
+// the 2- and 4-wave slots of 2048 and 4096 points 2 and 1, so that
+// workgroups that wait at each other's barriers stay small.
+inline int c2r_ramp_slots(int X, int YB) {
+  const int cap = X <= 1024 ? 4 : (X == 2048 ? 2 : 1);
+  return YB / 2 < cap ? YB / 2 : cap;
+}
+
+template <int X, int STORE, int P>
+static void launch_c2r_ramp_x(dim3 grid, const float2 *G, int Y, int XB,
+                              int ndf, int t0, int T, const float *wscale,
+                              const uint32_t *tmax, float nrm_mul, void *slab,
+                              int NYB, int YB, hipStream_t s,
+                              const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P>), grid,
+                     dim3(P * (X / default_e(X))), 0, s, G, Y, XB, ndf, t0, T,
+                     wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw);
+}
+
+template <int STORE, int P>
+static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
+                                    int ndf, int t0, int ntilts, int T,
+                                    const float *wscale, const uint32_t *tmax,
+                                    float nrm_mul, void *slab, int NYB, int YB,
+                                    hipStream_t s, const float2 *tw) {
+  dim3 grid((Y + 2 * P - 1) / (2 * P), ndf, ntilts);
+#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+  switch (X) {
+    case 256: launch_c2r_ramp_x<256, STORE, P>(A_); break;
+    case 512: launch_c2r_ramp_x<512, STORE, P>(A_); break;
+    case 1024: launch_c2r_ramp_x<1024, STORE, P>(A_); break;
+    case 2048:
+      if constexpr (P <= 2) {
+        launch_c2r_ramp_x<2048, STORE, P>(A_);
+        break;
+      }
+      return hipErrorInvalidValue;
     case 4096:
-      if constexpr (IT == 1) {
-        hipLaunchKernelGGL((k_rows_c2r_filter<4096, STORE, 1>), grid,
-                           dim3(2 * (2 * 4096 / default_e(2 * 4096))), 0, s,
-                           G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul,
-                           (ST *)slab, NYB, YB, tw);
+      if constexpr (P <= 1) {
+        launch_c2r_ramp_x<4096, STORE, P>(A_);
         break;
       }
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
+#undef A_
   return hipGetLastError();
 }
 
@@ -966,9 +1137,17 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
                                       hipStream_t s, const float2 *tw) {
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+  if (X >= 256 && YB >= 8) {
+    switch (c2r_ramp_slots(X, YB)) {
+      case 1: return launch_c2r_ramp_p<STORE, 1>(A_);
+      case 2: return launch_c2r_ramp_p<STORE, 2>(A_);
+      case 4: return launch_c2r_ramp_p<STORE, 4>(A_);
+    }
+    return hipErrorInvalidValue;
+  }
   const int rb = c2r_filter_rows(X, STORE, YB);
   if (YB % rb != 0) return hipErrorInvalidValue;
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
   if (rb == 4) return launch_c2r_filter_it<STORE, 1>(A_);
   if constexpr (STORE == kF32) {
     if (rb == 8) return launch_c2r_filter_it<STORE, 2>(A_);

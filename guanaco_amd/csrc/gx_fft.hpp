@@ -338,7 +338,8 @@ struct Fft {
   static_assert((N & (N - 1)) == 0, "N must be a power of two");
   static_assert(N <= kTwN, "N exceeds the twiddle table");
   static constexpr int NT = N / E;  // threads per transform
-  static_assert(!WAVE || NT == 64, "a wave-scope transform is one wave");
+  static_assert(!WAVE || (NT <= 64 && 64 % NT == 0),
+                "a wave-scope transform lies within one wave");
   static constexpr int NS = plan_stages(N, E);
   static_assert(NS == 1 || N >= E * E, "invalid end radix");
   static constexpr int kLds = fft_lds_len(N, E);
