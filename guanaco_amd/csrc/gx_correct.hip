@@ -318,9 +318,17 @@ __device__ __forceinline__ float2 load_pair_spectrum(const float2 *g, int X,
                                                      int kx) {
   const int kk = kx <= X / 2 ? kx : X - kx;
   const int cb = kk / CB, ci = kk % CB;
+  // Rows past the image load row Y - 1 and are zeroed afterwards: selecting
+  // values rather than addresses keeps these plain global loads (a select
+  // between the row and a local zero becomes a generic-address load of a
+  // selected pointer). One (tilt, defocus) spectrum holds fewer than 2^32
+  // elements, so the offsets are 32-bit.
+  const unsigned base = (unsigned)cb * (unsigned)Y;
+  const float2 A0 = g[(base + (unsigned)min(ya, Y - 1)) * CB + ci];
+  const float2 B0 = g[(base + (unsigned)min(yb, Y - 1)) * CB + ci];
   const float2 zero = make_float2(0.0f, 0.0f);
-  const float2 A = ya < Y ? g[((size_t)cb * Y + ya) * CB + ci] : zero;
-  const float2 B = yb < Y ? g[((size_t)cb * Y + yb) * CB + ci] : zero;
+  const float2 A = ya < Y ? A0 : zero;
+  const float2 B = yb < Y ? B0 : zero;
   return kx <= X / 2 ? make_float2(A.x - B.y, A.y + B.x)
                      : make_float2(A.x + B.y, B.x - A.y);
 }
@@ -482,8 +490,15 @@ __global__ __launch_bounds__(P *(X / default_e(X))) void k_rows_c2r_ramp(
   static_assert(E == 16, "sizes of 256 points and more");
   __shared__ float2 s[P * SL];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
-  const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
-  const int R0 = blockIdx.x * (2 * P);
+  // A 1D grid over (row group, defocus, tilt), remapped so that each XCD
+  // takes consecutive row groups: the RB-row runs of one slab row block, and
+  // the halves of the 128-byte spectrum lines that row groups share, then
+  // meet in that XCD's L2 instead of reaching HBM as partial lines.
+  const unsigned nry = (unsigned)(Y + 2 * P - 1) / (2 * P);
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int k = (int)((lb / nry) % (unsigned)ndf);
+  const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
+  const int R0 = (int)(lb % nry) * (2 * P);
   const int ya = R0 + 2 * p, yb = ya + 1;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
   float2 *ls = s + p * SL;
@@ -1105,7 +1120,9 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     const float *wscale, const uint32_t *tmax,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw) {
-  dim3 grid((Y + 2 * P - 1) / (2 * P), ndf, ntilts);
+  const size_t nb = (size_t)((Y + 2 * P - 1) / (2 * P)) * ndf * ntilts;
+  if (nb > 0xffffffffu) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nb);
 #define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
   switch (X) {
     case 256: launch_c2r_ramp_x<256, STORE, P>(A_); break;
