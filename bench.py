@@ -88,15 +88,27 @@ def percentile(a, q):
 
 # ---------------------------------------------------------------- GPU -----
 
+def _wavelength():
+    """The 300 kV electron wavelength (A), from the package's host code."""
+    import guanaco_amd.detail as ext
+    return ext.get_electron_wavelength(300000.0)
+
+
+def _plan_defoci(df, ndf, width, centre, ps):
+    """(min, max, step, n) of the automatic defocus range, from the
+    package's get_defocus_steps (the GPU legs use no oracle code)."""
+    from guanaco_amd.detail.correct import get_defocus_steps
+    return get_defocus_steps(df, ndf, None, width, centre, ps)
+
+
 def plan_kwargs(w, storage):
-    from oracle import oracle as O  # only for the wavelength constant
-    l = O.electron_wavelength(300000)
+    l = _wavelength()
     kw = dict(num_defocus=w["ndf"], pixel_size=w["ps"], centre=w["X"] / 2.0,
               storage=storage)
     if w["df"] is not None:
         centre = numpy.full(w["Y"], w["X"] / 2.0, numpy.float32)
-        mn, mx, st, n = _defocus_steps(w["df"], w["ndf"], w["X"], centre,
-                                       w["ps"])
+        mn, mx, st, n = _plan_defoci(w["df"], w["ndf"], w["X"], centre,
+                                     w["ps"])
         dfs = numpy.array([mn + d * st for d in range(n)], numpy.float64)
         kw.update(min_defocus=mn - w["df"], max_defocus=mx - w["df"],
                   defocus=dfs, wavelength=l, Cs=2.7e7, phi=w["phi"])
@@ -141,7 +153,7 @@ def served_path(C, w, angles, d_in, storage, n, max_inflight, log):
 
 
 def _defocus_steps(df, ndf, width, centre, ps):
-    # correct.py:44-93 (automatic step)
+    # correct.py:44-93 (automatic step); the oracle legs' own copy
     if ndf == 1:
         return df, df, 0, 1
     z0 = df - centre * ps
@@ -356,8 +368,7 @@ def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
     arrivals at 0.7 x capacity; latency is submission to completion."""
     import threading
     from guanaco_amd.serve import Server
-    from oracle import oracle as O  # only for the wavelength constant
-    l = O.electron_wavelength(300000)
+    l = _wavelength()
     ps = 1.5
     angles121 = numpy.deg2rad(numpy.arange(121) - 60.0).astype(numpy.float32)
     series = {}
@@ -381,7 +392,7 @@ def serving_bench(C, n_requests, stream, seed=1005, max_inflight=4,
         start = 60 - (T - 1) // 2
         view = C.DeviceView(series[S].ptr + start * S * S * 4, (T, S, S))
         centre = numpy.full(S, S / 2.0, numpy.float32)
-        mn, mx, st, n = _defocus_steps(df, ndf, S, centre, ps)
+        mn, mx, st, n = _plan_defoci(df, ndf, S, centre, ps)
         dfs = numpy.array([mn + d * st for d in range(n)], numpy.float64)
         kw = dict(num_defocus=ndf, pixel_size=ps, min_defocus=mn - df,
                   max_defocus=mx - df, storage="f16", defocus=dfs,
@@ -547,11 +558,10 @@ def main():
           file=sys.stderr, flush=True)
 
     # synthetic input, generated and kept in HBM
-    from oracle import oracle as O
     d_in = C.DeviceArray((T, Y, X))
     ctf = None
     if w["df"] is not None:
-        ctf = dict(wavelength=O.electron_wavelength(300000), Cs=2.7e7,
+        ctf = dict(wavelength=_wavelength(), Cs=2.7e7,
                    defocus=w["df"], phi=w["phi"], pixel_size=w["ps"])
     C.synth_projections(d_in, X, Y, angles, depth=Z, width=X,
                         num_ellipsoids=w["ne"], seed=w["seed"], semi=w["semi"],

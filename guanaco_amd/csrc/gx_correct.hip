@@ -455,7 +455,78 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
       for (int it = 0; it < IT; ++it) dp[it] = out[m][it];
     } else {
       const ST *src = reinterpret_cast<const ST *>(out[m]);
-      for (int r = 0; r < 2 * IT && r0 + r < Y; ++r) dst[r] = src[r];
+#pragma unroll
+      for (int r = 0; r < 2 * IT; ++r)  // constant indices keep out[] in VGPRs
+        if (r0 + r < Y) dst[r] = src[r];
+    }
+  }
+}
+
+// Writes a row group of the slab: P slots, slot p holding rows (R0 + 2p,
+// R0 + 2p + 1) of column tid + m NT in out[m] (E = X / NT per thread), of
+// which the first ncols columns are stored ([y / YB][x][y % YB], rows of
+// ncols). s: the workgroup's P slot regions of fft_lds_len(X, E) float2.
+// One slot, and the last, partial row group, store from registers; otherwise
+// the slots stage [column][slot] in LDS (stride P + 1 PT: odd in 4-byte
+// words for 16-bit storage, so a wave's column writes fall in distinct
+// banks), in NCH passes of XC columns when the regions do not hold all X
+// columns at once, and the workgroup writes each column's RB = 2P rows as
+// whole 4-, 8- or 16-byte runs.
+template <int X, int STORE, int P>
+__device__ __forceinline__ void store_row_group(
+    const typename RowPair<STORE>::T (&out)[default_e(X)], float2 *s, int p,
+    int tid, typename StoreT<STORE>::T *__restrict__ slab, int k, int t,
+    int R0, int Y, int ncols, int T, int NYB, int YB) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  using PT = typename RowPair<STORE>::T;
+  using ST = typename StoreT<STORE>::T;
+  const int ya = R0 + 2 * p, yb = ya + 1;
+  if (P == 1 || R0 + 2 * P > Y) {
+    if (ya >= Y) return;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * NT;
+      if (n >= ncols) continue;
+      ST *dst = slab + slab_index(k, t, ya, n, T, NYB, ncols, YB);
+      const ST *v = reinterpret_cast<const ST *>(&out[m]);
+      dst[0] = v[0];
+      if (yb < Y) dst[1] = v[1];
+    }
+    return;
+  }
+  constexpr int NCH = sizeof(PT) / 4;
+  constexpr int XC = X / NCH;
+  constexpr int XS = P + 1;
+  static_assert((size_t)XC * XS * sizeof(PT) <=
+                    (size_t)P * fft_lds_len(X, E) * sizeof(float2),
+                "staging fits");
+  constexpr int QW = P * sizeof(PT) < 16 ? P : 16 / (int)sizeof(PT);
+  constexpr int NQ = P / QW;  // runs per column
+  using RunT = typename std::conditional<
+      QW * sizeof(PT) == 16, uint4,
+      typename std::conditional<QW * sizeof(PT) == 8, uint2,
+                                uint32_t>::type>::type;
+  static_assert(QW * sizeof(PT) == sizeof(RunT), "whole runs");
+  PT *st = reinterpret_cast<PT *>(s);
+  ST *base = slab + slab_index(k, t, R0, 0, T, NYB, ncols, YB);
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    lds_barrier();  // the slots' transforms, or the previous pass, are done
+#pragma unroll
+    for (int m = c * (E / NCH); m < (c + 1) * (E / NCH); ++m)
+      st[(tid + m * NT - c * XC) * XS + p] = out[m];
+    lds_barrier();
+    for (int q = threadIdx.x; q < XC * NQ; q += P * NT) {
+      const int x = q / NQ, j = q % NQ;
+      if (c * XC + x >= ncols) break;  // q grows with x
+      PT v[QW];
+#pragma unroll
+      for (int i = 0; i < QW; ++i) v[i] = st[x * XS + j * QW + i];
+      RunT w;
+      __builtin_memcpy(&w, v, sizeof(RunT));
+      *reinterpret_cast<RunT *>(base + (size_t)(c * XC + x) * YB +
+                                2 * QW * j) = w;
     }
   }
 }
@@ -547,51 +618,89 @@ __global__ __launch_bounds__(P *(X / default_e(X))) void k_rows_c2r_ramp(
     out[m] = RowPair<STORE>::pack((e[m].x + o.x) * sc, (e[m].y + o.y) * sc);
   }
 
-  if (P == 1 || R0 + 2 * P > Y) {  // one slot, or the last, partial block
-    if (ya >= Y) return;
+  store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, X, T, NYB,
+                               YB);
+}
+
+// The volume path's filter (no correction) as four X-point transforms per
+// row pair, X = L / 2: for rows zero-padded to L (N <= X values each), the
+// even bins of FFT_L are FFT_X(z) and the odd bins FFT_X(z c), c(n) =
+// exp(-i pi n / X); after the ramp (table ramp[j], j <= X, 1/L included),
+// IFFT_L(U)[n] = IFFT_X(U[2i])[n] + conj(c(n)) IFFT_X(U[2i + 1])[n] for
+// n < N. Slots, workgroups and the slab stores as in k_rows_c2r_ramp.
+template <int X, int STORE, int P>
+__global__ __launch_bounds__(P *(X / default_e(X))) void k_filter_ramp(
+    const float *__restrict__ src, size_t st, size_t sk, size_t sy, int N,
+    int Y, int T, int ndf, const float *__restrict__ ramp,
+    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
+    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+    int YB, const float2 *__restrict__ tw) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  constexpr bool WAVE = NT <= 64;
+  constexpr int L = 2 * X;
+  constexpr int SL = fft_lds_len(X, E);
+  using PT = typename RowPair<STORE>::T;
+  static_assert(E == 16, "sizes of 256 points and more");
+  __shared__ float2 s[P * SL];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  // 1D grid over (row group, tilt, slab), XCD-aware as in k_rows_c2r_ramp
+  const unsigned nry = (unsigned)(Y + 2 * P - 1) / (2 * P);
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int t = (int)((lb / nry) % (unsigned)T);
+  const int k = (int)(lb / (nry * (unsigned)T));
+  const int R0 = (int)(lb % nry) * (2 * P);
+  const int ya = R0 + 2 * p, yb = ya + 1;
+  const float *base = src + t * st + k * sk;
+  const float *a = base + (size_t)min(ya, Y - 1) * sy;
+  const float *b = base + (size_t)min(yb, Y - 1) * sy;
+  float2 *ls = s + p * SL;
+
+  typename Fft<X, E, -1, 1, WAVE>::Twiddles twx;  // both directions
+  twx.load(tid, tw);
+  constexpr int kHalf = kTwN / L;  // W^(kHalf n) = c(n)
+  const float2 ct = tw[tid * kHalf];
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+
+  float2 z[E], w[E];
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      ST *dst = slab + slab_index(k, t, ya, tid + m * NT, T, NYB, X, YB);
-      const ST *v = reinterpret_cast<const ST *>(&out[m]);
-      dst[0] = v[0];
-      if (yb < Y) dst[1] = v[1];
-    }
-    return;
+  for (int m = 0; m < E; ++m) {
+    // clamped loads, then zeroed values (no select between addresses)
+    const int n = tid + m * NT;
+    const int nc = min(n, N - 1);
+    const float va = a[nc], vb = b[nc];
+    z[m] = make_float2(ya < Y && n < N ? va : 0.0f,
+                       yb < Y && n < N ? vb : 0.0f);
+    w[m] = cmul(z[m], cmul(ct, tw[m * (kTwN / (2 * E))]));
   }
-  // Stage [column][slot] (stride P + 1 PT: odd in 4-byte words for 16-bit
-  // storage, so a wave's column writes fall in distinct banks), in NCH passes
-  // of XC columns when the slots' regions do not hold all X columns at once.
-  constexpr int NCH = sizeof(PT) / 4;
-  constexpr int XC = X / NCH;
-  constexpr int XS = P + 1;
-  static_assert((size_t)XC * XS * sizeof(PT) <= sizeof(s), "staging fits");
-  constexpr int QW = P * sizeof(PT) < 16 ? P : 16 / (int)sizeof(PT);
-  constexpr int NQ = P / QW;  // runs per column
-  using RunT = typename std::conditional<
-      QW * sizeof(PT) == 16, uint4,
-      typename std::conditional<QW * sizeof(PT) == 8, uint2,
-                                uint32_t>::type>::type;
-  static_assert(QW * sizeof(PT) == sizeof(RunT), "whole runs");
-  PT *st = reinterpret_cast<PT *>(s);
-  ST *base = slab + slab_index(k, t, R0, 0, T, NYB, X, YB);
+  // even bins
+  Fft<X, E, -1, 1, WAVE>::run(z, ls, tid, twx);
 #pragma unroll
-  for (int c = 0; c < NCH; ++c) {
-    lds_barrier();  // the slots' transforms, or the previous pass, are done
-#pragma unroll
-    for (int m = c * (E / NCH); m < (c + 1) * (E / NCH); ++m)
-      st[(tid + m * NT - c * XC) * XS + p] = out[m];
-    lds_barrier();
-    for (int q = threadIdx.x; q < XC * NQ; q += P * NT) {
-      const int x = q / NQ, j = q % NQ;
-      PT v[QW];
-#pragma unroll
-      for (int i = 0; i < QW; ++i) v[i] = st[x * XS + j * QW + i];
-      RunT w;
-      __builtin_memcpy(&w, v, sizeof(RunT));
-      *reinterpret_cast<RunT *>(base + (size_t)(c * XC + x) * YB +
-                                2 * QW * j) = w;
-    }
+  for (int m = 0; m < E; ++m) {
+    const int j = 2 * (tid + m * NT);
+    const float r = ramp[j <= X ? j : L - j];
+    z[m] = make_float2(z[m].x * r, z[m].y * r);
   }
+  Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
+  // odd bins
+  Fft<X, E, -1, 1, WAVE>::run(w, ls, tid, twx);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int j = 2 * (tid + m * NT) + 1;
+    const float r = ramp[j <= X ? j : L - j];
+    w[m] = make_float2(w[m].x * r, w[m].y * r);
+  }
+  Fft<X, E, 1, 1, WAVE>::run(w, ls, tid, twx);
+  PT out[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const float2 c = cmul(ct, tw[m * (kTwN / (2 * E))]);
+    const float2 o = cmul(w[m], make_float2(c.x, -c.y));
+    out[m] = RowPair<STORE>::pack((z[m].x + o.x) * sc, (z[m].y + o.y) * sc);
+  }
+  store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, N, T, NYB,
+                               YB);
 }
 
 // Unfiltered complex-to-real rows for corr(): out[t][k][y][x] (float).
@@ -753,21 +862,32 @@ __global__ __launch_bounds__(2 * (L / default_e(L))) void k_filter_rows_blk(
       for (int it = 0; it < IT; ++it) dp[it] = out[m][it];
     } else {
       const ST *src2 = reinterpret_cast<const ST *>(out[m]);
-      for (int r = 0; r < 2 * IT && r0 + r < Y; ++r) dst[r] = src2[r];
+#pragma unroll
+      for (int r = 0; r < 2 * IT; ++r)  // constant indices keep out[] in VGPRs
+        if (r0 + r < Y) dst[r] = src2[r];
     }
   }
 }
 
-// max |row| per tilt over rows (k, t, y), for reduced-precision scaling.
+// max |row| per tilt over rows (k, t, y), for reduced-precision scaling:
+// one workgroup per (slab, tilt, group of kAbsRows rows), one atomic each.
+constexpr int kAbsRows = 16;
 __global__ void k_absmax_rows(const float *__restrict__ src, size_t st,
                               size_t sk, size_t sy, int N, int Y, int T,
                               int ndf, uint32_t *__restrict__ tmax) {
   __shared__ float red[4];
-  const long r = blockIdx.x;
-  const int y = (int)(r % Y), t = (int)((r / Y) % T), k = (int)(r / ((long)Y * T));
-  const float *a = src + t * st + k * sk + y * sy;
+  const int ng = (Y + kAbsRows - 1) / kAbsRows;
+  const long b = blockIdx.x;
+  const int g = (int)(b % ng), t = (int)((b / ng) % T);
+  const int k = (int)(b / ((long)ng * T));
+  const float *base = src + t * st + k * sk;
+  const int y0 = g * kAbsRows, y1 = min(Y, y0 + kAbsRows);
   float m = 0.0f;
-  for (int n = threadIdx.x; n < N; n += blockDim.x) m = fmaxf(m, fabsf(a[n]));
+  for (int y = y0; y < y1; ++y) {
+    const float *a = base + (size_t)y * sy;
+    for (int n = threadIdx.x; n < N; n += blockDim.x)
+      m = fmaxf(m, fabsf(a[n]));
+  }
   m = block_max<256>(m, red);
   if (threadIdx.x == 0) atomicMax(&tmax[t], __float_as_uint(m));
 }
@@ -1269,13 +1389,79 @@ static hipError_t launch_filter_blk_it(const float *src, size_t st, size_t sk,
                        Y, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
                        tw);                                                    \
     break;
-    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
-    C_(4096)
+    C_(8) C_(16) C_(32) C_(64) C_(128) C_(256)
 #undef C_
     default:
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int X, int STORE, int P>
+static void launch_filter_ramp_x(dim3 grid, const float *src, size_t st,
+                                 size_t sk, size_t sy, int N, int Y, int T,
+                                 int ndf, const float *ramp,
+                                 const float *wscale, const uint32_t *tmax,
+                                 float nrm_mul, void *slab, int NYB, int YB,
+                                 hipStream_t s, const float2 *tw) {
+  using ST = typename StoreT<STORE>::T;
+  hipLaunchKernelGGL((k_filter_ramp<X, STORE, P>), grid,
+                     dim3(P * (X / default_e(X))), 0, s, src, st, sk, sy, N,
+                     Y, T, ndf, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB,
+                     YB, tw);
+}
+
+template <int STORE, int P>
+static hipError_t launch_filter_ramp_p(const float *src, size_t st, size_t sk,
+                                       size_t sy, int N, int L, int Y, int T,
+                                       int ndf, const float *ramp,
+                                       const float *wscale,
+                                       const uint32_t *tmax, float nrm_mul,
+                                       void *slab, int NYB, int YB,
+                                       hipStream_t s, const float2 *tw) {
+  const size_t nb = (size_t)((Y + 2 * P - 1) / (2 * P)) * T * ndf;
+  if (nb > 0xffffffffu) return hipErrorInvalidValue;
+  dim3 grid((unsigned)nb);
+#define A_ grid, src, st, sk, sy, N, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+  switch (L / 2) {
+    case 256: launch_filter_ramp_x<256, STORE, P>(A_); break;
+    case 512: launch_filter_ramp_x<512, STORE, P>(A_); break;
+    case 1024: launch_filter_ramp_x<1024, STORE, P>(A_); break;
+    case 2048:
+      if constexpr (P <= 2) {
+        launch_filter_ramp_x<2048, STORE, P>(A_);
+        break;
+      }
+      return hipErrorInvalidValue;
+    case 4096:
+      if constexpr (P <= 1) {
+        launch_filter_ramp_x<4096, STORE, P>(A_);
+        break;
+      }
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef A_
+  return hipGetLastError();
+}
+
+template <int STORE>
+static hipError_t launch_filter_ramp_t(const float *src, size_t st, size_t sk,
+                                       size_t sy, int N, int L, int Y, int T,
+                                       int ndf, const float *ramp,
+                                       const float *wscale,
+                                       const uint32_t *tmax, float nrm_mul,
+                                       void *slab, int NYB, int YB,
+                                       hipStream_t s, const float2 *tw) {
+#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+  switch (c2r_ramp_slots(L / 2, YB)) {
+    case 1: return launch_filter_ramp_p<STORE, 1>(A_);
+    case 2: return launch_filter_ramp_p<STORE, 2>(A_);
+    case 4: return launch_filter_ramp_p<STORE, 4>(A_);
+  }
+#undef A_
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
@@ -1286,10 +1472,21 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  // row blocks of whole 16-byte runs where the slab layout allows them
-  // (the 8192-point transforms keep the four-row form)
+  // 512- to 8192-point filters: four half-length transforms per row pair
+  if (L >= 512 && L <= 8192 && YB >= 8) {
+#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+    switch (store) {
+      case kF32: return launch_filter_ramp_t<kF32>(A_);
+      case kF16: return launch_filter_ramp_t<kF16>(A_);
+      case kBF16: return launch_filter_ramp_t<kBF16>(A_);
+    }
+#undef A_
+    return hipErrorInvalidValue;
+  }
+  // shorter filters: row blocks of whole 16-byte runs where the slab layout
+  // allows them
   const int rb = c2r_filter_rows(L / 2, store, YB);
-  if (L <= 4096 && YB >= 8 && YB % rb == 0) {
+  if (L <= 256 && YB >= 8 && YB % rb == 0) {
 #define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
     if (store == kF32) {
       if (rb == 8) return launch_filter_blk_it<kF32, 2>(A_);
@@ -1323,8 +1520,9 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
 hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int Y, int T, int ndf,
                               uint32_t *tmax, hipStream_t s) {
-  const long nrows = (long)ndf * T * Y;
-  hipLaunchKernelGGL(k_absmax_rows, dim3((unsigned)nrows), dim3(256), 0, s,
+  const long nb = (long)ndf * T * ((Y + kAbsRows - 1) / kAbsRows);
+  if (nb > 0xffffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_absmax_rows, dim3((unsigned)nb), dim3(256), 0, s,
                      src, st, sk, sy, N, Y, T, ndf, tmax);
   return hipGetLastError();
 }
