@@ -214,6 +214,65 @@ def test_volume_sinogram_layout_and_weights(capi):
     assert rel_l2(got, expect) < 1e-5
 
 
+# Rows of 256 points and more take the four-transform row passes
+# (k_rows_c2r_ramp with the correction, k_filter_ramp without): one, two and
+# four row pairs per workgroup (X = 4096, 2048, <= 1024), row counts that
+# leave a partial last row group, and a row shorter than its transform
+# (N = 300 filtered on the 1024-point grid, 512-point halves).
+RAMP_CTF_CASES = [
+    # X, Y, T, ndf, Z, ps, df
+    (256, 12, 7, 1, 8, 1.0, 20000.0),
+    (512, 9, 5, 2, 8, 1.0, 25000.0),
+    (1024, 16, 4, 1, 8, 2.0, 20000.0),
+    (2048, 6, 3, 2, 4, 1.5, 30000.0),
+    (4096, 3, 3, 1, 4, 1.0, 25000.0),
+]
+
+
+@pytest.mark.parametrize("case", RAMP_CTF_CASES)
+def test_volume_ctf_long_rows_match_oracle_chain(capi, case):
+    X, Y, T, ndf, Z, ps, df = case
+    proj, angles, centre, dfs, rel = ctf_problem(X, Y, T, ndf, ps, df, X + Y)
+    corrected, mn, mx = O.correct_projections(
+        proj, centre, ps, 300, df, ndf, None, 2.7, 0, 0, 0)
+    expect = O.reconstruct(corrected, angles, X / 2.0, ps, mn, mx, depth=Z)
+    got = gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, "f32", dfs)
+    assert rel_l2(got, expect) < 1e-5
+    corrected, mn, mx = O.correct_projections(
+        proj, centre, ps, 300, df, ndf, None, 2.7, 0, 0, 0, precision="f64")
+    expect = O.reconstruct(corrected, angles, X / 2.0, ps, mn, mx, depth=Z,
+                           precision="f64")
+    for storage, bound in (("f16", 1e-3), ("bf16", 4e-3)):
+        got = gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, storage, dfs)
+        assert rel_l2(got, expect) < bound, storage
+
+
+RAMP_PLAIN_CASES = [
+    # X, Y, T, Z
+    (256, 13, 7, 16),
+    (300, 10, 6, 16),
+    (1024, 8, 5, 8),
+    (2048, 5, 3, 4),
+    (4096, 3, 3, 4),
+]
+
+
+@pytest.mark.parametrize("case", RAMP_PLAIN_CASES)
+def test_volume_plain_long_rows_match_oracle(capi, case):
+    X, Y, T, Z = case
+    rng = numpy.random.default_rng(X + Y)
+    proj = rng.standard_normal((T, Y, X)).astype(numpy.float32)
+    angles = tilts(T)
+    got = gpu_volume(proj, angles, X, Y, Z, 1, 1.0, (0.0, 0.0), "f32")
+    expect = O.reconstruct(proj, angles, X / 2.0, 1.0, depth=Z)
+    assert rel_l2(got, expect) < 1e-5
+    expect = O.reconstruct(proj, angles, X / 2.0, 1.0, depth=Z,
+                           precision="f64")
+    for storage, bound in (("f16", 1e-3), ("bf16", 4e-3)):
+        got = gpu_volume(proj, angles, X, Y, Z, 1, 1.0, (0.0, 0.0), storage)
+        assert rel_l2(got, expect) < bound, storage
+
+
 # ------------------------------------------- graphs and synthetic data ----
 
 def test_graph_replays_match_eager(capi):
