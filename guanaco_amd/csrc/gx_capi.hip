@@ -671,7 +671,7 @@ extern "C" int gx_corr(const float *image, const float *ctf, float *rec,
   const gx::BluesteinTable by =
       bluestein_view(d_tab.as<float2>() + ytab, ysize, MY);
   GX_HIP(launch_mask_from_ctf(d_mask.as<int8_t>(), d_ctf.as<float2>(), X, Y, XB,
-                              ndf, s));
+                              ndf, cp, s));
   if (rp) {
     GX_HIP(launch_rows_r2c(d_img.as<float>(), npix, X, Y, XB, 1,
                            d_F.as<float2>(), nullptr, 0, s));
@@ -1026,7 +1026,8 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
                           sizeof(double) * p->ndf, hipMemcpyHostToDevice, s));
     GX_HIP(launch_ctf_mask(p->mask.as<int8_t>(), p->X, p->Y, p->XB, p->ndf,
                            p->defocus.as<double>(), p->wavelength, p->Cs,
-                           p->Ca, p->Pa, p->phi, p->pixel_size, s));
+                           p->Ca, p->Pa, p->phi, p->pixel_size, !p->cols_bs,
+                           s));
   }
   return GX_OK;
 }

@@ -75,12 +75,25 @@ __device__ double chi_at(int x, int y, int w, int h, double inv_ps, double l,
   return M_PI * (Cs * l * l * l * q * q * q * q / 2 - l * d * q * q);
 }
 
+// Where the flip-mask value of (defocus k, column block cb, row ky, column
+// ci) lies. The power-of-two column pass (k_cols) reads it in thread order,
+// [k][cb][tid][ci][m] with ky = tid + m NT (NT = Y / E threads per column):
+// the E values one thread multiplies for one defocus are one E-byte load.
+// The Bluestein column pass reads [k][cb][ky][ci].
+__host__ __device__ __forceinline__ size_t mask_offset(int k, int cb, int ky,
+                                                       int ci, int XB, int Y,
+                                                       bool thread_order) {
+  if (!thread_order) return (((size_t)k * XB + cb) * Y + ky) * CB + ci;
+  const int E = default_e(Y), NT = Y / E;
+  return ((((size_t)k * XB + cb) * NT + ky % NT) * CB + ci) * E + ky / NT;
+}
+
 // mask[k][cb][ky][ci] = M(k) + M(-k) in {-2..2}: twice the Hermitian part of
 // the flip, which is what Re(IFFT2(...)) keeps. Columns past X/2 are 0.
 __global__ void k_ctf_mask(int8_t *__restrict__ mask, int X, int Y, int XB,
                            int ndf, const double *__restrict__ defocus,
                            double l, double Cs, double Ca, double Pa,
-                           double phi, double inv_ps) {
+                           double phi, double inv_ps, bool thread_order) {
   const size_t total = (size_t)ndf * XB * Y * CB;
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e >= total) return;
@@ -102,17 +115,17 @@ __global__ void k_ctf_mask(int8_t *__restrict__ mask, int X, int Y, int XB,
     }
     v = m1 + m2;
   }
-  mask[e] = (int8_t)v;
+  mask[mask_offset(k, cb, ky, ci, XB, Y, thread_order)] = (int8_t)v;
 }
 
 hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
                            const double *d_defocus, double l, double Cs,
                            double Ca, double Pa, double phi, double ps,
-                           hipStream_t s) {
+                           bool thread_order, hipStream_t s) {
   const size_t total = (size_t)ndf * XB * Y * CB;
   hipLaunchKernelGGL(k_ctf_mask, dim3((unsigned)((total + 255) / 256)),
                      dim3(256), 0, s, mask, X, Y, XB, ndf, d_defocus, l, Cs,
-                     Ca, Pa, phi, 1.0 / ps);
+                     Ca, Pa, phi, 1.0 / ps, thread_order);
   return hipGetLastError();
 }
 
@@ -120,7 +133,7 @@ hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
 // reference's corr() receives it (ext/correct.h:32-76).
 __global__ void k_mask_from_ctf(int8_t *__restrict__ mask,
                                 const float2 *__restrict__ ctf, int X, int Y,
-                                int XB, int ndf) {
+                                int XB, int ndf, bool thread_order) {
   const size_t total = (size_t)ndf * XB * Y * CB;
   const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (e >= total) return;
@@ -137,14 +150,15 @@ __global__ void k_mask_from_ctf(int8_t *__restrict__ mask,
     const float b = c[(size_t)my * X + mx].y;
     v = -((0.0f < a) - (a < 0.0f)) - ((0.0f < b) - (b < 0.0f));
   }
-  mask[e] = (int8_t)v;
+  mask[mask_offset(k, cb, ky, ci, XB, Y, thread_order)] = (int8_t)v;
 }
 
 hipError_t launch_mask_from_ctf(int8_t *mask, const float2 *ctf, int X, int Y,
-                                int XB, int ndf, hipStream_t s) {
+                                int XB, int ndf, bool thread_order,
+                                hipStream_t s) {
   const size_t total = (size_t)ndf * XB * Y * CB;
   hipLaunchKernelGGL(k_mask_from_ctf, dim3((unsigned)((total + 255) / 256)),
-                     dim3(256), 0, s, mask, ctf, X, Y, XB, ndf);
+                     dim3(256), 0, s, mask, ctf, X, Y, XB, ndf, thread_order);
   return hipGetLastError();
 }
 
@@ -294,13 +308,36 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[(size_t)(tid + m * NT) * CB + ci];
   Fft<Y, E, -1, CB>::run(v, s + ci, tid, twy);
+  // int8 flip masks in thread order (mask_offset): this thread's E values of
+  // one defocus are one E-byte load (loading a defocus ahead measured slower:
+  // the extra registers spill)
+  using MaskRun = typename std::conditional<
+      E == 16, uint4,
+      typename std::conditional<
+          E == 8, uint2,
+          typename std::conditional<E == 4, uint32_t, uint16_t>::type>::type>::
+      type;
+  constexpr bool kRun = sizeof(M) == 1 && sizeof(MaskRun) == E;
+  const size_t run0 = ((size_t)cb * NT + tid) * CB + ci;  // in E-byte runs
   for (int k = 0; k < ndf; ++k) {
-    const M *mk = mask + ((size_t)k * XB + cb) * Y * CB;
     float2 w[E];
+    if constexpr (kRun) {
+      int8_t mv[E];
+      const MaskRun cur = reinterpret_cast<const MaskRun *>(
+            mask)[(size_t)k * XB * NT * CB + run0];
+      __builtin_memcpy(mv, &cur, E);
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const float f = mask_value(mk[(size_t)(tid + m * NT) * CB + ci]);
-      w[m] = make_float2(v[m].x * f, v[m].y * f);
+      for (int m = 0; m < E; ++m) {
+        const float f = mask_value(mv[m]);
+        w[m] = make_float2(v[m].x * f, v[m].y * f);
+      }
+    } else {
+      const M *mk = mask + ((size_t)k * XB + cb) * Y * CB;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const float f = mask_value(mk[(size_t)(tid + m * NT) * CB + ci]);
+        w[m] = make_float2(v[m].x * f, v[m].y * f);
+      }
     }
     Fft<Y, E, 1, CB>::run(w, s + ci, tid, twy);
     float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;

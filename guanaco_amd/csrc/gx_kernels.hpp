@@ -79,12 +79,15 @@ hipError_t launch_ctf_grid(float2 *out, int w, int h, double ps,
                            const double *d_defocus, int ndf,
                            const CtfParams &c, bool envelope, bool recentre,
                            hipStream_t s);
+// thread_order: the layout of the power-of-two column pass (k_cols), else
+// the Bluestein column pass's [k][cb][ky][ci]
 hipError_t launch_ctf_mask(int8_t *mask, int X, int Y, int XB, int ndf,
                            const double *d_defocus, double l, double Cs,
                            double Ca, double Pa, double phi, double ps,
-                           hipStream_t s);
+                           bool thread_order, hipStream_t s);
 hipError_t launch_mask_from_ctf(int8_t *mask, const float2 *ctf, int X, int Y,
-                                int XB, int ndf, hipStream_t s);
+                                int XB, int ndf, bool thread_order,
+                                hipStream_t s);
 
 hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
                            int XB, int ntilts, float2 *F, uint32_t *tmax,
