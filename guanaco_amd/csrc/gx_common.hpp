@@ -57,12 +57,24 @@ struct StoreT<kF16> {
     return __half2float(v);
   }
 };
+// gfx950 converts to bfloat16 in hardware (v_cvt_pk_bf16_f32, two values per
+// instruction), rounding to nearest even: the same bits as f32_to_bf16_bits
+// for every finite value.
+__device__ __forceinline__ uint32_t bf16x2_bits(float a, float b) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  const bf16x2 r = __builtin_convertvector((f32x2){a, b}, bf16x2);
+  uint32_t u;
+  __builtin_memcpy(&u, &r, 4);
+  return u;
+}
+
 template <>
 struct StoreT<kBF16> {
   using T = bf16_t;
   __device__ __forceinline__ static T pack(float f) {
     bf16_t b;
-    b.u = f32_to_bf16_bits(f);
+    b.u = (uint16_t)bf16x2_bits(f, 0.0f);
     return b;
   }
   __device__ __forceinline__ static float unpack(T v) {

@@ -400,7 +400,7 @@ template <>
 struct RowPair<kBF16> {
   using T = uint32_t;
   __device__ __forceinline__ static T pack(float a, float b) {
-    return f32_to_bf16_bits(a) | (f32_to_bf16_bits(b) << 16);
+    return bf16x2_bits(a, b);
   }
 };
 
@@ -583,7 +583,8 @@ __device__ __forceinline__ void store_row_group(
 // column, so that the workgroup writes each column's RB rows as whole
 // 8- or 16-byte runs.
 template <int X, int STORE, int P>
-__global__ __launch_bounds__(P *(X / default_e(X))) void k_rows_c2r_ramp(
+__global__ __launch_bounds__(P *(X / default_e(X)))
+__attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
     const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
@@ -665,8 +666,11 @@ __global__ __launch_bounds__(P *(X / default_e(X))) void k_rows_c2r_ramp(
 // exp(-i pi n / X); after the ramp (table ramp[j], j <= X, 1/L included),
 // IFFT_L(U)[n] = IFFT_X(U[2i])[n] + conj(c(n)) IFFT_X(U[2i + 1])[n] for
 // n < N. Slots, workgroups and the slab stores as in k_rows_c2r_ramp.
+// Both kernels are held to three waves per SIMD (<= 168 VGPRs): the
+// 2048-point, two-slot forms otherwise take 178 and run two.
 template <int X, int STORE, int P>
-__global__ __launch_bounds__(P *(X / default_e(X))) void k_filter_ramp(
+__global__ __launch_bounds__(P *(X / default_e(X)))
+__attribute__((amdgpu_waves_per_eu(3))) void k_filter_ramp(
     const float *__restrict__ src, size_t st, size_t sk, size_t sy, int N,
     int Y, int T, int ndf, const float *__restrict__ ramp,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
