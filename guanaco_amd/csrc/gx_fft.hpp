@@ -235,17 +235,24 @@ __device__ __forceinline__ void fft_sync() {
 //
 // Middle stage S multiplies element m of each group by W^(m k kTwN/(Ns R)),
 // k = b mod Ns. Every b = tid + q N/E of a thread has the same k (Ns divides
-// N/E), so a stage needs R - 1 values. The last stage needs W^(m tid kTwN/N)
-// for m < E: formed as lo[m mod 4] * hi[m / 4] (one rounding) from at most
-// three values of each kind.
+// N/E), so a stage needs R - 1 values; a radix-16 stage keeps six, W^(j ...)
+// and W^(4j ...) for j = 1..3, and forms the others as their products (one
+// rounding), which saves nine registers per such stage. The last stage needs
+// W^(m tid kTwN/N) for m < E: formed the same way as lo[m mod 4] * hi[m / 4]
+// from at most three values of each kind.
 //
 // The device table holds W^j (kTwN entries) followed, for every (N, E), by
 // these per-thread values laid out [value][tid], so that each load of a
 // wave is one contiguous run.
+// values kept for a middle stage of radix R
+__host__ __device__ constexpr int tw_mid_count(int R) {
+  return R == 16 ? 6 : R - 1;
+}
 __host__ __device__ constexpr int tw_mid_before(int N, int E, int S) {
   int c = 0;
   const int ns = plan_stages(N, E);
-  for (int s = 1; s < S && s < ns - 1; ++s) c += plan_radix(N, E, s) - 1;
+  for (int s = 1; s < S && s < ns - 1; ++s)
+    c += tw_mid_count(plan_radix(N, E, s));
   return c;
 }
 __host__ __device__ constexpr int tw_lo_count(int E) { return E - 1 < 3 ? E - 1 : 3; }
@@ -298,6 +305,21 @@ struct FftTwiddles {
       for (int i = 0; i < kHi; ++i) hi[i] = t[(kMid + kLo + i) * NT];
     }
   }
+  // middle-stage factor m (0 < m < R) of the stage whose values start at
+  // mid[m0]: kept as is, or for R = 16 the product lo(m mod 4) hi(m / 4)
+  template <int R>
+  __device__ __forceinline__ static float2 mid_at(const float2 *mid, int m0,
+                                                  int m) {
+    #pragma clang fp contract(fast)
+    if constexpr (R == 16) {
+      const int m1 = m & 3, m2 = m >> 2;
+      if (m2 == 0) return mid[m0 + m1 - 1];
+      if (m1 == 0) return mid[m0 + 3 + m2 - 1];
+      return cmul(mid[m0 + m1 - 1], mid[m0 + 3 + m2 - 1]);
+    } else {
+      return mid[m0 + m - 1];
+    }
+  }
   // last-stage factor for element m (0 < m < E)
   __device__ __forceinline__ float2 last(int m) const {
     #pragma clang fp contract(fast)
@@ -317,8 +339,16 @@ inline void tw_fill_block(int N, int E, const float2 *w, float2 *out) {
     for (int s = 1; s < ns - 1; ++s) {
       const int R = plan_radix(N, E, s), Ns = plan_ns(N, E, s);
       const int TWS = kTwN / (Ns * R), k = tid & (Ns - 1);
-      for (int m = 1; m < R; ++m)
-        out[(tw_mid_before(N, E, s) + m - 1) * NT + tid] = w[m * k * TWS];
+      const int base = tw_mid_before(N, E, s);
+      if (R == 16) {
+        for (int j = 1; j <= 3; ++j) {
+          out[(base + j - 1) * NT + tid] = w[j * k * TWS];
+          out[(base + 3 + j - 1) * NT + tid] = w[4 * j * k * TWS];
+        }
+      } else {
+        for (int m = 1; m < R; ++m)
+          out[(base + m - 1) * NT + tid] = w[m * k * TWS];
+      }
     }
     const int TWS = kTwN / N;
     for (int m = 1; m <= nlo; ++m)
@@ -373,7 +403,9 @@ struct Fft {
         const int b = tid + q * NT;
 #pragma unroll
         for (int m = 1; m < R; ++m)
-          w[q * R + m] = cmul(w[q * R + m], dir_conj<DIR>(tw.mid[M0 + m - 1]));
+          w[q * R + m] =
+              cmul(w[q * R + m], dir_conj<DIR>(Twiddles::template mid_at<R>(
+                                     tw.mid, M0, m)));
         dft<R, DIR>(&w[q * R]);
         const int sbase = slot((b - k) * R + k);
 #pragma unroll
