@@ -521,14 +521,17 @@ __device__ __forceinline__ void store_row_group(
   const int ya = R0 + 2 * p, yb = ya + 1;
   if (P == 1 || R0 + 2 * P > Y) {
     if (ya >= Y) return;
+    const bool pair = yb < Y;  // ya is even: the pair is one aligned PT
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
       if (n >= ncols) continue;
       ST *dst = slab + slab_index(k, t, ya, n, T, NYB, ncols, YB);
-      const ST *v = reinterpret_cast<const ST *>(&out[m]);
-      dst[0] = v[0];
-      if (yb < Y) dst[1] = v[1];
+      if (pair) {
+        *reinterpret_cast<PT *>(dst) = out[m];
+      } else {
+        dst[0] = reinterpret_cast<const ST *>(&out[m])[0];
+      }
     }
     return;
   }
