@@ -585,47 +585,22 @@ __device__ __forceinline__ void store_row_group(
 // row blocking) then meet once, to stage their outputs in LDS column by
 // column, so that the workgroup writes each column's RB rows as whole
 // 8- or 16-byte runs.
-template <int X, int STORE, int P>
-__global__ __launch_bounds__(P *(X / default_e(X)))
-__attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
-    const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
-    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
-    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-    int YB, const float2 *__restrict__ tw) {
+// One row pair (ya, yb) of one (tilt, defocus) spectrum g through the four
+// transforms: out[m] = rows (ya, yb) of column tid + m NT, times sc, packed.
+// ls: the pair's X-point LDS region; twx: this thread's X-point twiddles;
+// ct = c(tid).
+template <int X, int STORE, bool WAVE>
+__device__ __forceinline__ void ramp_pair(
+    const float2 *g, int Y, int ya, int yb, float2 *ls, int tid,
+    const typename Fft<X, default_e(X), 1, 1, WAVE>::Twiddles &twx,
+    float2 ct, const float2 *__restrict__ tw, float sc,
+    typename RowPair<STORE>::T (&out)[default_e(X)]) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
-  constexpr bool WAVE = NT <= 64;
   constexpr int L = 2 * X;
-  constexpr int SL = fft_lds_len(X, E);
-  using PT = typename RowPair<STORE>::T;
-  using ST = typename StoreT<STORE>::T;
-  static_assert(E == 16, "sizes of 256 points and more");
-  __shared__ float2 s[P * SL];
-  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
-  // A 1D grid over (row group, defocus, tilt), remapped so that each XCD
-  // takes consecutive row groups: the RB-row runs of one slab row block, and
-  // the halves of the 128-byte spectrum lines that row groups share, then
-  // meet in that XCD's L2 instead of reaching HBM as partial lines.
-  const unsigned nry = (unsigned)(Y + 2 * P - 1) / (2 * P);
-  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int k = (int)((lb / nry) % (unsigned)ndf);
-  const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
-  const int R0 = (int)(lb % nry) * (2 * P);
-  const int ya = R0 + 2 * p, yb = ya + 1;
-  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
-  float2 *ls = s + p * SL;
-
-  typename Fft<X, E, 1, 1, WAVE>::Twiddles twx;  // both directions
-  twx.load(tid, tw);
-  // c(tid + m NT) = c(tid) exp(-i pi m / E); both factors from the W^j table
-  constexpr int kHalf = kTwN / L;  // W^(kHalf n) = c(n)
-  const float2 ct = tw[tid * kHalf];
-  const float sc =
-      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
   // filter.cpp:30-45: r[j] = |j| / X on the 2X grid, times the 1/L of the
   // unnormalised inverse (powers of two: exact as the reference's product)
   constexpr float kRamp = 1.0f / ((float)X * (float)L);
-
   float2 z[E], e[E];
 #pragma unroll
   for (int m = 0; m < E; ++m)
@@ -651,26 +626,97 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
     z[m] = make_float2(z[m].x * r, z[m].y * r);
   }
   Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
-  PT out[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const float2 c = cmul(ct, tw[m * (kTwN / (2 * E))]);
     const float2 o = cmul(z[m], make_float2(c.x, -c.y));
     out[m] = RowPair<STORE>::pack((e[m].x + o.x) * sc, (e[m].y + o.y) * sc);
   }
-
-  store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, X, T, NYB,
-                               YB);
 }
 
-// The volume path's filter (no correction) as four X-point transforms per
-// row pair, X = L / 2: for rows zero-padded to L (N <= X values each), the
-// even bins of FFT_L are FFT_X(z) and the odd bins FFT_X(z c), c(n) =
-// exp(-i pi n / X); after the ramp (table ramp[j], j <= X, 1/L included),
-// IFFT_L(U)[n] = IFFT_X(U[2i])[n] + conj(c(n)) IFFT_X(U[2i + 1])[n] for
-// n < N. Slots, workgroups and the slab stores as in k_rows_c2r_ramp.
-// Both kernels are held to three waves per SIMD (<= 168 VGPRs): the
-// 2048-point, two-slot forms otherwise take 178 and run two.
+// The workgroup's P slots each take one row pair (IT = 1), or its one slot
+// takes IT = 2 pairs in turn (16-bit storage at 4096 points, where a slot
+// is the whole 256-thread workgroup): the first pair's packed rows wait in
+// an LDS stash (X words, read back by the thread that wrote them) rather
+// than in registers, and each column's four rows leave as one 8-byte write.
+template <int X, int STORE, int P, int IT>
+__global__ __launch_bounds__(P *(X / default_e(X)))
+__attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
+    const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
+    const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
+    float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
+    int YB, const float2 *__restrict__ tw) {
+  constexpr int E = default_e(X);
+  constexpr int NT = X / E;
+  constexpr bool WAVE = NT <= 64;
+  constexpr int L = 2 * X;
+  constexpr int SL = fft_lds_len(X, E);
+  constexpr int RB = 2 * P * IT;  // rows per workgroup
+  using PT = typename RowPair<STORE>::T;
+  using ST = typename StoreT<STORE>::T;
+  static_assert(E == 16, "sizes of 256 points and more");
+  static_assert(IT == 1 || (IT == 2 && P == 1 && sizeof(PT) == 4),
+                "two pairs per slot: one slot, 16-bit storage");
+  __shared__ float2 s[P * SL];
+  __shared__ PT stash[IT > 1 ? X : 1];
+  const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
+  // A 1D grid over (row group, defocus, tilt), remapped so that each XCD
+  // takes consecutive row groups: the RB-row runs of one slab row block, and
+  // the halves of the 128-byte spectrum lines that row groups share, then
+  // meet in that XCD's L2 instead of reaching HBM as partial lines.
+  const unsigned nry = (unsigned)(Y + RB - 1) / RB;
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int k = (int)((lb / nry) % (unsigned)ndf);
+  const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
+  const int R0 = (int)(lb % nry) * RB;
+  const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
+  float2 *ls = s + p * SL;
+
+  typename Fft<X, E, 1, 1, WAVE>::Twiddles twx;  // both directions
+  twx.load(tid, tw);
+  // c(tid + m NT) = c(tid) exp(-i pi m / E); both factors from the W^j table
+  constexpr int kHalf = kTwN / L;  // W^(kHalf n) = c(n)
+  const float2 ct = tw[tid * kHalf];
+  const float sc =
+      wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
+
+  PT out[E];
+  if constexpr (IT == 1) {
+    ramp_pair<X, STORE, WAVE>(g, Y, R0 + 2 * p, R0 + 2 * p + 1, ls, tid, twx,
+                              ct, tw, sc, out);
+    store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, X, T,
+                                 NYB, YB);
+  } else {
+    ramp_pair<X, STORE, WAVE>(g, Y, R0, R0 + 1, ls, tid, twx, ct, tw, sc, out);
+#pragma unroll
+    for (int m = 0; m < E; ++m) stash[tid + m * NT] = out[m];
+    // The second pair recomputes its per-column factors (c(n), ramp)
+    // instead of keeping the first pair's: held across its transforms they
+    // would take 64 more registers.
+    int tid2 = tid;
+    float2 ct2 = ct;
+    asm volatile("" : "+v"(tid2), "+v"(ct2)::"memory");
+    ramp_pair<X, STORE, WAVE>(g, Y, R0 + 2, R0 + 3, ls, tid2, twx, ct2, tw, sc,
+                              out);
+    if (R0 >= Y) return;
+    const bool whole = R0 + 4 <= Y;  // R0 % 4 == 0: the run is 8-byte aligned
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int n = tid + m * NT;
+      ST *dst = slab + slab_index(k, t, R0, n, T, NYB, X, YB);
+      const uint2 w = make_uint2(stash[n], out[m]);
+      if (whole) {
+        *reinterpret_cast<uint2 *>(dst) = w;
+      } else {
+        const ST *v = reinterpret_cast<const ST *>(&w);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (R0 + r < Y) dst[r] = v[r];
+      }
+    }
+  }
+}
+
 template <int X, int STORE, int P>
 __global__ __launch_bounds__(P *(X / default_e(X)))
 __attribute__((amdgpu_waves_per_eu(3))) void k_filter_ramp(
@@ -1266,14 +1312,14 @@ inline int c2r_ramp_slots(int X, int YB) {
   return YB / 2 < cap ? YB / 2 : cap;
 }
 
-template <int X, int STORE, int P>
+template <int X, int STORE, int P, int IT = 1>
 static void launch_c2r_ramp_x(dim3 grid, const float2 *G, int Y, int XB,
                               int ndf, int t0, int T, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, void *slab,
                               int NYB, int YB, hipStream_t s,
                               const float2 *tw) {
   using ST = typename StoreT<STORE>::T;
-  hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P>), grid,
+  hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P, IT>), grid,
                      dim3(P * (X / default_e(X))), 0, s, G, Y, XB, ndf, t0, T,
                      wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw);
 }
@@ -1284,7 +1330,9 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     const float *wscale, const uint32_t *tmax,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw) {
-  const size_t nb = (size_t)((Y + 2 * P - 1) / (2 * P)) * ndf * ntilts;
+  // two row pairs per slot at 4096 points with 16-bit storage
+  const int rb = X == 4096 && STORE != kF32 ? 4 : 2 * P;
+  const size_t nb = (size_t)((Y + rb - 1) / rb) * ndf * ntilts;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
 #define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
@@ -1299,8 +1347,11 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
       }
       return hipErrorInvalidValue;
     case 4096:
-      if constexpr (P <= 1) {
-        launch_c2r_ramp_x<4096, STORE, P>(A_);
+      if constexpr (P == 1 && STORE != kF32) {
+        launch_c2r_ramp_x<4096, STORE, 1, 2>(A_);
+        break;
+      } else if constexpr (P == 1) {
+        launch_c2r_ramp_x<4096, STORE, 1>(A_);
         break;
       }
       return hipErrorInvalidValue;
