@@ -252,7 +252,9 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
     v[m] = make_float2(a, b);
     amax = fmaxf(amax, fmaxf(fabsf(a), fabsf(b)));
   }
-  Fft<X, E, -1, 1>::run(v, s[p], tid, tw);
+  // a pair of rows is one wave (or part of one) up to 1024 points: its
+  // transform synchronises only that wave
+  Fft<X, E, -1, 1, (NT <= 64)>::run(v, s[p], tid, tw);
 #pragma unroll
   for (int m = 0; m < E; ++m) s[p][tid + m * NT] = v[m];
   lds_barrier();
@@ -634,10 +636,12 @@ __device__ __forceinline__ void ramp_pair(
   }
 }
 
-// The workgroup's P slots each take one row pair (IT = 1), or its one slot
-// takes IT = 2 pairs in turn (16-bit storage at 4096 points, where a slot
-// is the whole 256-thread workgroup): the first pair's packed rows wait in
-// an LDS stash (X words, read back by the thread that wrote them) rather
+// The workgroup's P slots each take one row pair (IT = 1), or two pairs in
+// turn (IT = 2, 16-bit storage at 4096 points, where the one slot is the
+// whole 256-thread workgroup; at 2048 points it measured slower with 32-row
+// slabs): slot p then takes
+// rows R0 + 4p .. R0 + 4p + 3, the first pair's packed rows wait in an LDS
+// stash (X words per slot, read back by the thread that wrote them) rather
 // than in registers, and each column's four rows leave as one 8-byte write.
 template <int X, int STORE, int P, int IT>
 __global__ __launch_bounds__(P *(X / default_e(X)))
@@ -655,10 +659,10 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
   using PT = typename RowPair<STORE>::T;
   using ST = typename StoreT<STORE>::T;
   static_assert(E == 16, "sizes of 256 points and more");
-  static_assert(IT == 1 || (IT == 2 && P == 1 && sizeof(PT) == 4),
-                "two pairs per slot: one slot, 16-bit storage");
+  static_assert(IT == 1 || (IT == 2 && sizeof(PT) == 4),
+                "two pairs per slot: 16-bit storage");
   __shared__ float2 s[P * SL];
-  __shared__ PT stash[IT > 1 ? X : 1];
+  __shared__ PT stash[IT > 1 ? P * X : 1];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   // A 1D grid over (row group, defocus, tilt), remapped so that each XCD
   // takes consecutive row groups: the RB-row runs of one slab row block, and
@@ -687,31 +691,33 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
     store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, X, T,
                                  NYB, YB);
   } else {
-    ramp_pair<X, STORE, WAVE>(g, Y, R0, R0 + 1, ls, tid, twx, ct, tw, sc, out);
+    const int Rs = R0 + 4 * p;  // this slot's four rows
+    PT *sp = stash + p * X;
+    ramp_pair<X, STORE, WAVE>(g, Y, Rs, Rs + 1, ls, tid, twx, ct, tw, sc, out);
 #pragma unroll
-    for (int m = 0; m < E; ++m) stash[tid + m * NT] = out[m];
+    for (int m = 0; m < E; ++m) sp[tid + m * NT] = out[m];
     // The second pair recomputes its per-column factors (c(n), ramp)
     // instead of keeping the first pair's: held across its transforms they
     // would take 64 more registers.
     int tid2 = tid;
     float2 ct2 = ct;
     asm volatile("" : "+v"(tid2), "+v"(ct2)::"memory");
-    ramp_pair<X, STORE, WAVE>(g, Y, R0 + 2, R0 + 3, ls, tid2, twx, ct2, tw, sc,
+    ramp_pair<X, STORE, WAVE>(g, Y, Rs + 2, Rs + 3, ls, tid2, twx, ct2, tw, sc,
                               out);
-    if (R0 >= Y) return;
-    const bool whole = R0 + 4 <= Y;  // R0 % 4 == 0: the run is 8-byte aligned
+    if (Rs >= Y) return;
+    const bool whole = Rs + 4 <= Y;  // Rs % 4 == 0: the run is 8-byte aligned
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
-      ST *dst = slab + slab_index(k, t, R0, n, T, NYB, X, YB);
-      const uint2 w = make_uint2(stash[n], out[m]);
+      ST *dst = slab + slab_index(k, t, Rs, n, T, NYB, X, YB);
+      const uint2 w = make_uint2(sp[n], out[m]);
       if (whole) {
         *reinterpret_cast<uint2 *>(dst) = w;
       } else {
         const ST *v = reinterpret_cast<const ST *>(&w);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (R0 + r < Y) dst[r] = v[r];
+          if (Rs + r < Y) dst[r] = v[r];
       }
     }
   }
@@ -1331,7 +1337,7 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw) {
   // two row pairs per slot at 4096 points with 16-bit storage
-  const int rb = X == 4096 && STORE != kF32 ? 4 : 2 * P;
+  const int rb = X == 4096 && STORE != kF32 ? 4 * P : 2 * P;
   const size_t nb = (size_t)((Y + rb - 1) / rb) * ndf * ntilts;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
