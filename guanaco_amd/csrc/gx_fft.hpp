@@ -235,18 +235,19 @@ __device__ __forceinline__ void fft_sync() {
 //
 // Middle stage S multiplies element m of each group by W^(m k kTwN/(Ns R)),
 // k = b mod Ns. Every b = tid + q N/E of a thread has the same k (Ns divides
-// N/E), so a stage needs R - 1 values; a radix-16 stage keeps six, W^(j ...)
-// and W^(4j ...) for j = 1..3, and forms the others as their products (one
-// rounding), which saves nine registers per such stage. The last stage needs
+// N/E), so a stage needs R - 1 values; a radix-8 or -16 stage keeps W^(j ...)
+// for j = 1..3 and W^(4j ...) for j < R/4, and forms the others as their
+// products (one rounding), which saves 6 registers (R = 8) or 18 (R = 16). The last stage needs
 // W^(m tid kTwN/N) for m < E: formed the same way as lo[m mod 4] * hi[m / 4]
 // from at most three values of each kind.
 //
 // The device table holds W^j (kTwN entries) followed, for every (N, E), by
 // these per-thread values laid out [value][tid], so that each load of a
 // wave is one contiguous run.
-// values kept for a middle stage of radix R
+// values kept for a middle stage of radix R: W^j (j = 1..3) and W^(4j)
+// (j = 1..R/4 - 1) for R = 8 and 16
 __host__ __device__ constexpr int tw_mid_count(int R) {
-  return R == 16 ? 6 : R - 1;
+  return R >= 8 ? 3 + R / 4 - 1 : R - 1;
 }
 __host__ __device__ constexpr int tw_mid_before(int N, int E, int S) {
   int c = 0;
@@ -306,12 +307,12 @@ struct FftTwiddles {
     }
   }
   // middle-stage factor m (0 < m < R) of the stage whose values start at
-  // mid[m0]: kept as is, or for R = 16 the product lo(m mod 4) hi(m / 4)
+  // mid[m0]: kept as is, or for R = 8, 16 the product lo(m mod 4) hi(m / 4)
   template <int R>
   __device__ __forceinline__ static float2 mid_at(const float2 *mid, int m0,
                                                   int m) {
     #pragma clang fp contract(fast)
-    if constexpr (R == 16) {
+    if constexpr (R >= 8) {
       const int m1 = m & 3, m2 = m >> 2;
       if (m2 == 0) return mid[m0 + m1 - 1];
       if (m1 == 0) return mid[m0 + 3 + m2 - 1];
@@ -340,11 +341,10 @@ inline void tw_fill_block(int N, int E, const float2 *w, float2 *out) {
       const int R = plan_radix(N, E, s), Ns = plan_ns(N, E, s);
       const int TWS = kTwN / (Ns * R), k = tid & (Ns - 1);
       const int base = tw_mid_before(N, E, s);
-      if (R == 16) {
-        for (int j = 1; j <= 3; ++j) {
-          out[(base + j - 1) * NT + tid] = w[j * k * TWS];
+      if (R >= 8) {
+        for (int j = 1; j <= 3; ++j) out[(base + j - 1) * NT + tid] = w[j * k * TWS];
+        for (int j = 1; j < R / 4; ++j)
           out[(base + 3 + j - 1) * NT + tid] = w[4 * j * k * TWS];
-        }
       } else {
         for (int m = 1; m < R; ++m)
           out[(base + m - 1) * NT + tid] = w[m * k * TWS];
