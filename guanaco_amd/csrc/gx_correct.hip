@@ -293,23 +293,43 @@ __device__ __forceinline__ float mask_value(M m) {
   }
 }
 
-template <int Y, typename M>
-__global__ __launch_bounds__(CB *(Y / default_e(Y)))
+// CW columns per workgroup (CB, or CB / 2 for 4096-point columns, whose
+// 4-column workgroups fill a CU alone: two then overlap each other's
+// barriers; C4 filter 340 -> 327 ms, while at 2048 points, where two
+// 4-column workgroups already share a CU, halving them was slower). With
+// CW < CB the grid
+// is 1D over (column half, column block, tilt), remapped so that each XCD
+// takes consecutive ones: the two halves of a column block, which share its
+// 32-byte row segments, meet in one L2.
+template <int Y, typename M, int CW>
+__global__ __launch_bounds__(CW *(Y / default_e(Y)))
 __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
     const float2 *__restrict__ F, const M *__restrict__ mask,
     float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw) {
   constexpr int E = default_e(Y);
   constexpr int NT = Y / E;
-  __shared__ float2 s[fft_lds_len(Y, E) * CB];
-  const int ci = threadIdx.x % CB, tid = threadIdx.x / CB;
-  const int cb = blockIdx.x, t = blockIdx.y;
+  constexpr int H = CB / CW;  // workgroups per column block
+  static_assert(CB % CW == 0, "whole column blocks");
+  __shared__ float2 s[fft_lds_len(Y, E) * CW];
+  const int cl = threadIdx.x % CW, tid = threadIdx.x / CW;
+  int cb, t, ci;
+  if constexpr (H == 1) {
+    cb = blockIdx.x;
+    t = blockIdx.y;
+    ci = cl;
+  } else {
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    ci = (int)(lb % H) * CW + cl;
+    cb = (int)((lb / H) % (unsigned)XB);
+    t = (int)(lb / (H * (unsigned)XB));
+  }
   const float2 *src = F + ((size_t)t * XB + cb) * Y * CB;
-  typename Fft<Y, E, -1, CB>::Twiddles twy;  // shared by every transform here
+  typename Fft<Y, E, -1, CW>::Twiddles twy;  // shared by every transform here
   twy.load(tid, tw);
   float2 v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = src[(size_t)(tid + m * NT) * CB + ci];
-  Fft<Y, E, -1, CB>::run(v, s + ci, tid, twy);
+  Fft<Y, E, -1, CW>::run(v, s + cl, tid, twy);
   // int8 flip masks in thread order (mask_offset): this thread's E values of
   // one defocus are one E-byte load (loading a defocus ahead measured slower:
   // the extra registers spill)
@@ -341,7 +361,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
         w[m] = make_float2(v[m].x * f, v[m].y * f);
       }
     }
-    Fft<Y, E, 1, CB>::run(w, s + ci, tid, twy);
+    Fft<Y, E, 1, CW>::run(w, s + cl, tid, twy);
     float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
 #pragma unroll
     for (int m = 0; m < E; ++m) dst[(size_t)(tid + m * NT) * CB + ci] = w[m];
@@ -1250,13 +1270,24 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
   dim3 grid(XB, ntilts);
+  const dim3 grid2((unsigned)(2 * XB * ntilts));
   switch (Y) {
 #define C_(n)                                                              \
   case n:                                                                  \
-    hipLaunchKernelGGL((k_cols<n, M>), grid, dim3(CB * (n / default_e(n))), \
-                       0, s, F, mask, G, XB, ndf, tw);                     \
+    hipLaunchKernelGGL((k_cols<n, M, CB>), grid,                           \
+                       dim3(CB * (n / default_e(n))), 0, s, F, mask, G, XB, \
+                       ndf, tw);                                           \
     break;
-    GX_POW2_CASES(C_)
+    C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
+    C_(2048)
+#undef C_
+#define C_(n)                                                              \
+  case n:                                                                  \
+    hipLaunchKernelGGL((k_cols<n, M, CB / 2>), grid2,                      \
+                       dim3(CB / 2 * (n / default_e(n))), 0, s, F, mask, G, \
+                       XB, ndf, tw);                                       \
+    break;
+    C_(4096)
 #undef C_
     default:
       return hipErrorInvalidValue;
