@@ -95,6 +95,10 @@ CORR_CASES = [
     (64, 32, 1.5, [30000.0, 31000.0], 400.0, 0.4, 0.1),
     (256, 128, 1.0, [25000.0], 0.0, 0.0, math.asin(0.1)),
     (1024, 1024, 2.0, [20000.0], 0.0, 0.0, math.asin(0.1)),
+    # long columns: 4-column (2048) and 2-column (4096) workgroups, several
+    # defoci through the thread-order flip mask
+    (2048, 32, 1.5, [30000.0, 30500.0], 0.0, 0.0, 0.1),
+    (4096, 16, 1.0, [20000.0, 21000.0, 22000.0], 300.0, 0.3, 0.1),
 ]
 
 
@@ -226,6 +230,8 @@ RAMP_CTF_CASES = [
     (1024, 16, 4, 1, 8, 2.0, 20000.0),
     (2048, 6, 3, 2, 4, 1.5, 30000.0),
     (4096, 3, 3, 1, 4, 1.0, 25000.0),
+    # 4096-point columns (2-column workgroups) under short rows
+    (64, 4096, 2, 2, 4, 1.0, 25000.0),
 ]
 
 
