@@ -308,6 +308,66 @@ def oracle_check(w, proj, angles, rows, vols, pool, dump=None):
     return out
 
 
+def _oracle_slice(a):
+    """One y-slice of filter + backprojection through the oracle (a pool
+    worker's share of the all-cores baseline); returns seconds."""
+    from oracle import oracle as O
+    sino, Z, X, angles, ps = a
+    t0 = time.perf_counter()
+    O.recon(sino, (Z, X), angles, X / 2.0, ps)
+    return time.perf_counter() - t0
+
+
+_WORKER_CTF = {}
+
+
+def _oracle_correction(a):
+    """One tilt's correction through the oracle; returns seconds. The CTF
+    grid is built once per worker process from its parameters."""
+    from oracle import oracle as O
+    img, key = a
+    if key not in _WORKER_CTF:
+        df, phi, X, Y, ps = key
+        _WORKER_CTF[key] = O.ctf_simple_grid(
+            O.electron_wavelength(300000), df, 2.7e7, 0, 0, phi, X, Y,
+            ps).astype(numpy.complex64)
+    t0 = time.perf_counter()
+    O.corr(img, _WORKER_CTF[key], precision="f32")
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_all_cores(w, proj, angles, pool, n):
+    """The oracle slice-parallel on the pool's workers (the upper bound of a
+    cpu_count-process run): two rounds of y-slices and one of tilt
+    corrections, timed by wall clock and extrapolated to the whole job."""
+    T, Y, X, Z = w["T"], w["Y"], w["X"], w["Z"]
+    t_corr = 0.0
+    if w["df"] is not None:
+        key = (w["df"], w["phi"], X, Y, w["ps"])
+        k = min(n, T)
+        tasks = [(proj[t], key) for t in range(k)]
+        # an untimed round first, so that the workers' CTF grids are built
+        pool.map(_oracle_correction, tasks, chunksize=1)
+        t0 = time.perf_counter()
+        pool.map(_oracle_correction, tasks, chunksize=1)
+        t_corr = (time.perf_counter() - t0) / k
+    ys = numpy.linspace(0, Y - 1, min(2 * n, Y)).astype(int)
+    t0 = time.perf_counter()
+    pool.map(_oracle_slice, [(numpy.ascontiguousarray(proj[:, y, :]), Z, X,
+                              angles, w["ps"]) for y in ys], chunksize=1)
+    t_slice = (time.perf_counter() - t0) / len(ys)
+    total = t_corr * T + t_slice * Y
+    return {
+        "value": X * Y * Z * T / total,
+        "unit": UNIT,
+        "cores": n,
+        "sample": "%d tilt corrections and %d y-slices on %d worker "
+                  "processes, wall clock, extrapolated linearly" % (
+                      min(n, T) if w["df"] is not None else 0, len(ys), n),
+        "seconds_estimated_whole_job": total,
+    }
+
+
 def cpu_baseline(w, proj, angles, budget_s):
     """Oracle (one core) on a bounded sample: some tilt corrections and some
     y-slices of filter + backprojection; extrapolated to the whole job."""
@@ -540,9 +600,11 @@ def main():
     check = not args.no_check and args.only is None and w.get("check", True)
     # CPU workers for the oracle check are forked before anything touches
     # the GPU, so they inherit no device state
+    # (and the all-cores CPU baseline's)
     pool = None
-    if check:
-        pool = mp.get_context("fork").Pool(min(16, os.cpu_count() or 1))
+    workers = min(16, os.cpu_count() or 1)
+    if check or not args.no_cpu:
+        pool = mp.get_context("fork").Pool(workers)
 
     from guanaco_amd import _capi as C
     if C.device_count() < 1:
@@ -709,7 +771,6 @@ def main():
         line["rel_l2"] = oracle_check(
             w, proj, angles, rows, {"f32": v32, args.storage: v16}, pool,
             dump=args.dump)
-        pool.close()
         line["rel_l2"]["sample"] = "y-slices %s" % rows.tolist()
         print("bench: oracle check %.1f s %s" % (time.perf_counter() - t0,
                                                 line["rel_l2"]),
@@ -729,6 +790,15 @@ def main():
     if not args.no_cpu:
         proj = d_in.to_host()
         line["cpu_baseline"] = cpu_baseline(w, proj, angles, args.cpu_budget)
+        line["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(
+            w, proj, angles, pool, workers)
+        print("bench: CPU oracle %.3g updates/s on 1 core, %.3g on %d" % (
+            line["cpu_baseline"]["value"],
+            line["cpu_baseline"]["all_cores"]["value"],
+            line["cpu_baseline"]["all_cores"]["cores"]), file=sys.stderr,
+            flush=True)
+    if pool is not None:
+        pool.close()
     for p in plans.values():
         p.destroy()
     print(json.dumps(line), flush=True)
