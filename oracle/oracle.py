@@ -1,8 +1,9 @@
 """CPU oracle for the guanaco reconstruction hot path (Python side).
 
-TEST INFRASTRUCTURE ONLY. tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` import this module as the checker; the
-product (``guanaco_amd``) never imports it.
+TEST INFRASTRUCTURE ONLY. tests/, ``__graft_entry__.smoke()`` and the CPU
+legs of ``bench.py`` (the ``cpu_baseline`` and the check of the GPU volumes
+against this oracle) import this module as the checker; the product
+(``guanaco_amd``) and the bench's GPU legs never import it.
 
 It wraps ``liboracle.so`` (the C restatement of the reference's CPU path, see
 guanaco_oracle.h) with ctypes, and restates the reference's pure-Python
