@@ -603,14 +603,12 @@ __device__ __forceinline__ void store_row_group(
 // r[2i + 1] FFT_X(c z)), r the ramp of the 2X grid. Every transform is X
 // points: a slot (one row pair) needs X / 16 threads and an X-point LDS
 // region, and at X <= 1024 a slot lies within one wave and synchronises
-// only itself. The P slots of a workgroup (RB = 2P rows dividing the slab's
-// row blocking) then meet once, to stage their outputs in LDS column by
-// column, so that the workgroup writes each column's RB rows as whole
-// 8- or 16-byte runs.
-// One row pair (ya, yb) of one (tilt, defocus) spectrum g through the four
-// transforms: out[m] = rows (ya, yb) of column tid + m NT, times sc, packed.
-// ls: the pair's X-point LDS region; twx: this thread's X-point twiddles;
-// ct = c(tid).
+// only itself (the workgroups and their slab stores: k_rows_c2r_ramp).
+//
+// ramp_pair: one row pair (ya, yb) of one (tilt, defocus) spectrum g through
+// the four transforms: out[m] = rows (ya, yb) of column tid + m NT, times sc,
+// packed. ls: the pair's X-point LDS region; twx: this thread's X-point
+// twiddles; ct = c(tid).
 template <int X, int STORE, bool WAVE>
 __device__ __forceinline__ void ramp_pair(
     const float2 *g, int Y, int ya, int yb, float2 *ls, int tid,
@@ -656,13 +654,16 @@ __device__ __forceinline__ void ramp_pair(
   }
 }
 
-// The workgroup's P slots each take one row pair (IT = 1), or two pairs in
+// The workgroup's P slots each take one row pair (IT = 1) and meet once, to
+// stage their outputs in LDS column by column (store_row_group), so that the
+// workgroup writes each column's RB = 2P rows (RB dividing the slab's row
+// blocking) as whole 8- or 16-byte runs. Or each slot takes two pairs in
 // turn (IT = 2, 16-bit storage at 4096 points, where the one slot is the
-// whole 256-thread workgroup; at 2048 points it measured slower with 32-row
-// slabs): slot p then takes
-// rows R0 + 4p .. R0 + 4p + 3, the first pair's packed rows wait in an LDS
-// stash (X words per slot, read back by the thread that wrote them) rather
-// than in registers, and each column's four rows leave as one 8-byte write.
+// whole 256-thread workgroup; at 1024 and 2048 points it measured slower):
+// slot p then takes rows R0 + 4p .. R0 + 4p + 3, the first pair's packed
+// rows wait in an LDS stash (X words per slot, read back by the thread that
+// wrote them) rather than in registers, and each column's four rows leave
+// as one 8-byte write.
 template <int X, int STORE, int P, int IT>
 __global__ __launch_bounds__(P *(X / default_e(X)))
 __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
