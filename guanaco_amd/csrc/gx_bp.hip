@@ -143,73 +143,44 @@ __device__ __forceinline__ int tap_index(float x, float z, float4 cf,
 }
 
 // reconstructor.cpp:66-73: slab k = clamp(floor(h * dscale + doffset + 0.5)),
-// h = -x s - z c. EXACT: the reference's float arithmetic, and floor((double)
-// d + 0.5) without FP64 (round half to even, then move exact ties up).
-// Otherwise (reduced storage) d is
-// two FMAs with the per-tilt factors sd = (-s dscale, -c dscale, doffset)
-// of the reduced coefficient table's second half, and the slab is
-// floor(d + 1/2) in float: slab boundaries may move by an ulp of d, and the
-// direct kernel, the staged kernel and its window table all use this form.
-template <bool EXACT>
-__device__ __forceinline__ int slab_of(float x, float z, float4 cf, float4 sd,
+// h = -x s - z c. floor((double)d + 0.5) without FP64: round half to even,
+// then move exact ties up.
+__device__ __forceinline__ int slab_of(float x, float z, float4 cf,
                                        float dscale, float doffset, int ndf) {
-  if constexpr (EXACT) {
-    const float h = -x * cf.y - z * cf.x;
-    const float d = h * dscale + doffset;
-    float n = rintf(d);
-    if (d - n == 0.5f) n += 1.0f;
-    return min(max((int)n, 0), ndf - 1);
-  } else {
-    #pragma clang fp contract(fast)
-    const float d = fmaf(x, sd.x, fmaf(z, sd.y, sd.z));
-    return min(max((int)floorf(d + 0.5f), 0), ndf - 1);
-  }
-}
-
-// The slab choice's per-tilt factors for slab_of: the second half of the
-// reduced coefficient table (unused by the exact form).
-template <bool EXACT>
-__device__ __forceinline__ float4 slab_factors(const float4 *coef, int T,
-                                               int t) {
-  if constexpr (EXACT) {
-    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  } else {
-    return kload(coef, (size_t)bp_tilt_pad(T) + t);
-  }
+  const float h = -x * cf.y - z * cf.x;
+  const float d = h * dscale + doffset;
+  float n = rintf(d);
+  if (d - n == 0.5f) n += 1.0f;
+  return min(max((int)n, 0), ndf - 1);
 }
 
 // ------------------------------------------------------- per-tilt tables --
 
 // out[t] = (c, s, offset - 1/2, scale * normalisation): the normalisation
 // float(tmax[t]) * nrm_mul is what the filter pass divided the stored values
-// by (1 where it is not positive, or without tmax); out[Tpad + t] = (-s
-// dscale, -c dscale, doffset, 0), the slab choice's factors. Entries past T
-// are zero.
+// by (1 where it is not positive, or without tmax). Entries past T are zero.
 __global__ void k_bp_coefficients(const float4 *__restrict__ coef,
                                   const uint32_t *__restrict__ tmax,
-                                  float nrm_mul, int T, int Tpad, float dscale,
-                                  float doffset, float4 *__restrict__ out) {
+                                  float nrm_mul, int T, int Tpad,
+                                  float4 *__restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= Tpad) return;
   float4 r = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float4 sd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (t < T) {
     const float4 cf = coef[t];
     float nrm = tmax != nullptr ? __uint_as_float(tmax[t]) * nrm_mul : 1.0f;
     if (!(nrm > 0.0f)) nrm = 1.0f;
     r = make_float4(cf.x, cf.y, cf.z - 0.5f, cf.w * nrm);
-    sd = make_float4(-cf.y * dscale, -cf.x * dscale, doffset, 0.0f);
   }
   out[t] = r;
-  out[Tpad + t] = sd;
 }
 
 hipError_t launch_bp_coefficients(const float4 *coef, const uint32_t *tmax,
-                                  float nrm_mul, int T, float dscale,
-                                  float doffset, float4 *out, hipStream_t s) {
+                                  float nrm_mul, int T, float4 *out,
+                                  hipStream_t s) {
   const int Tpad = bp_tilt_pad(T);
   hipLaunchKernelGGL(k_bp_coefficients, dim3((Tpad + 63) / 64), dim3(64), 0, s,
-                     coef, tmax, nrm_mul, T, Tpad, dscale, doffset, out);
+                     coef, tmax, nrm_mul, T, Tpad, out);
   return hipGetLastError();
 }
 
@@ -263,10 +234,9 @@ struct Staging {
 // Window of one tilt over the tile's active voxels [x0, x1] x [z0, z1]:
 // smallest i0 and smallest k over the four corners.
 template <bool EXACT, bool MULTI>
-__device__ __forceinline__ int2 tile_window(float4 cf, float4 sd, float x0,
-                                            float x1, float z0, float z1,
-                                            float dscale, float doffset,
-                                            int ndf) {
+__device__ __forceinline__ int2 tile_window(float4 cf, float x0, float x1,
+                                            float z0, float z1, float dscale,
+                                            float doffset, int ndf) {
   int w = 0x7fffffff, k = 0x7fffffff;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
@@ -274,8 +244,7 @@ __device__ __forceinline__ int2 tile_window(float4 cf, float4 sd, float x0,
     const float z = (c & 2) ? z1 : z0;
     float tt;
     w = min(w, tap_index<EXACT>(x, z, cf, tt));
-    if constexpr (MULTI)
-      k = min(k, slab_of<EXACT>(x, z, cf, sd, dscale, doffset, ndf));
+    if constexpr (MULTI) k = min(k, slab_of(x, z, cf, dscale, doffset, ndf));
   }
   return make_int2(w, MULTI ? k : 0);
 }
@@ -296,11 +265,8 @@ __global__ void k_bp_windows(const float4 *__restrict__ coef, int T, int Tpad,
     const float x1 = voxel_coord(min(bx * kTX + kTX, W) - 1, W);
     const float z0 = voxel_coord(bz * kTZ, Z);
     const float z1 = voxel_coord(min(bz * kTZ + kTZ, Z) - 1, Z);
-    // the reduced table's second half: the slab choice's factors
-    const float4 sd = EXACT ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
-                            : coef[Tpad + t];
-    r = tile_window<EXACT, MULTI>(coef[t], sd, x0, x1, z0, z1, dscale,
-                                  doffset, ndf);
+    r = tile_window<EXACT, MULTI>(coef[t], x0, x1, z0, z1, dscale, doffset,
+                                  ndf);
   }
   win[i] = r;
 }
@@ -384,9 +350,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
   for (int t = 0; t < T; ++t) {
     const float4 cf = kload(coef, t);
     int k = 0;
-    if constexpr (MULTI)
-      k = slab_of<kExact>(x, z, cf, slab_factors<kExact>(coef, T, t), dscale,
-                          doffset, ndf);
+    if constexpr (MULTI) k = slab_of(x, z, cf, dscale, doffset, ndf);
     float tt;
     const int i0 = tap_index<kExact>(x, z, cf, tt);
     if (active && (unsigned)i0 < (unsigned)(N - 1)) {
@@ -441,9 +405,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_rows(
   for (int t = 0; t < T; ++t) {
     const float4 cf = kload(coef, t);
     int k = 0;
-    if constexpr (MULTI)
-      k = slab_of<kExact>(x, z, cf, slab_factors<kExact>(coef, T, t), dscale,
-                          doffset, ndf);
+    if constexpr (MULTI) k = slab_of(x, z, cf, dscale, doffset, ndf);
     const typename ST::T *base =
         yblock + (size_t)t * NYB * row_len + (size_t)k * kstride;
     const float xc = x * cf.x - z * cf.y;  // kF32: the reference's first part
@@ -573,9 +535,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
         const int2 w = kload(tw, t);
         int ks = 0;
         if constexpr (MULTI) {
-          ks = slab_of<kExact>(x, z, cf, slab_factors<kExact>(coef, T, t),
-                               dscale, doffset, ndf) -
-               w.y;
+          ks = slab_of(x, z, cf, dscale, doffset, ndf) - w.y;
           ks = min(max(ks, 0), KS - 1);
         }
         float tt;

@@ -878,8 +878,7 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
     ps->f_bytes = ps->g_bytes = 0;
   }
   ps->chunk = (int)chunk;
-  // coef (Tpad float4) and coef_r (2 Tpad: constants and slab factors)
-  ps->small_bytes = (size_t)bp_tilt_pad((int)T) * 3 * sizeof(float4) +
+  ps->small_bytes = (size_t)bp_tilt_pad((int)T) * 2 * sizeof(float4) +
                     T * 2 * sizeof(float) +
                     bp_window_entries((int)d->width, (int)d->depth, (int)T) *
                         sizeof(int2) +
@@ -976,7 +975,7 @@ void plan_buffers(gx_volume_plan *p, const PlanSizes &ps,
   const size_t T = p->T, Tpad = (size_t)bp_tilt_pad(p->T);
   out = {{&p->slab, ps.slab_bytes},
          {&p->coef, sizeof(float4) * Tpad},
-         {&p->coef_r, sizeof(float4) * 2 * Tpad},
+         {&p->coef_r, sizeof(float4) * Tpad},
          {&p->win, sizeof(int2) * bp_window_entries(p->W, p->Z, p->T)},
          {&p->wscale, sizeof(float) * T},
          {&p->tmax, sizeof(uint32_t) * T},
@@ -1001,7 +1000,6 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
                              s));
   } else {
     GX_HIP(launch_bp_coefficients(p->coef.as<float4>(), nullptr, 1.0f, p->T,
-                                  p->dscale, p->doffset,
                                   p->coef_r.as<float4>(), s));
     GX_HIP(launch_bp_windows(p->coef_r.as<float4>(), 0, p->T, p->W, p->Z,
                              p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
@@ -1186,8 +1184,7 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
     a.coef = p->coef.as<float4>();
   } else {
     GX_HIP(launch_bp_coefficients(p->coef.as<float4>(), p->tmax.as<uint32_t>(),
-                                  p->nrm_mul, p->T, p->dscale, p->doffset,
-                                  p->coef_r.as<float4>(), s));
+                                  p->nrm_mul, p->T, p->coef_r.as<float4>(), s));
     a.coef = p->coef_r.as<float4>();
   }
   a.win = p->win.as<int2>();

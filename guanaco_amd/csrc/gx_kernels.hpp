@@ -120,8 +120,8 @@ hipError_t launch_bp(const BpArgs &a, int YB, int store, hipStream_t s);
 // Reduced-storage coefficient table from the reference's (c, s, offset,
 // scale) and the per-tilt normalisation of the filter pass.
 hipError_t launch_bp_coefficients(const float4 *coef, const uint32_t *tmax,
-                                  float nrm_mul, int T, float dscale,
-                                  float doffset, float4 *out, hipStream_t s);
+                                  float nrm_mul, int T, float4 *out,
+                                  hipStream_t s);
 // Window table of the staged kernel: bp_window_entries(W, Z, T) int2, from
 // the coefficient table the kernels read (exact != 0: kF32 arithmetic).
 size_t bp_window_entries(int W, int Z, int T);
