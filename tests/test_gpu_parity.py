@@ -462,6 +462,64 @@ def test_staged_bp_equals_direct_reduced_storage(capi, case, storage):
     plan.destroy()
 
 
+def slab_rows_16(plan, case, ys, storage):
+    """slab_rows for 16-bit slabs, decoded to float32."""
+    X, Y, T, ndf = case[:4]
+    p, n, yb = plan.slabs()
+    nyb = (Y + yb - 1) // yb
+    used = ndf * T * nyb * X * yb * 2
+    assert used <= n
+    a = numpy.empty(used // 2, numpy.uint16)
+    C.call("gx_memcpy_dtoh", a.ctypes.data, p, used, None)
+    if storage == "f16":
+        f = a.view(numpy.float16).astype(numpy.float32)
+    else:
+        f = (a.astype(numpy.uint32) << 16).view(numpy.float32)
+    f = f.reshape(ndf, T, nyb, X, yb)
+    return {y: numpy.ascontiguousarray(f[:, :, y // yb, :, y % yb])
+            for y in ys}
+
+
+# C3's geometry at two rows: 2048 wide, 1.5 A/px, +-1536 A in 16 slabs,
+# 61 tilts of 2 degrees, where rounding puts voxels exactly on slab ties
+SLAB_TIE_CASE = (2048, 2, 61, 16, 512, 2048, (-60.0, 60.0), (-1536.0, 1536.0),
+                 1.5)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("case", [BP_KIND_CASES[2], BP_KIND_CASES[3],
+                                  SLAB_TIE_CASE])
+def test_reduced_storage_bp_takes_the_reference_slabs(capi, case, storage):
+    # The slab of a (voxel, tilt) is a discontinuous choice, and on the
+    # half-integer voxel grid whole rows of voxels sit exactly on slab
+    # boundaries at some tilts: reduced storage must choose as the reference
+    # does. Backprojecting the plan's own decoded slabs (times the per-tilt
+    # normalisation the filter divided out: max |input| of the tilt) through
+    # the oracle leaves only FP32 rounding differences: below 1e-5 for narrow
+    # detectors, and at 2048 positions those of the reduced path's FMA tap
+    # geometry (an ulp of p is 1.2e-4): 1.35e-4 measured with the
+    # reference's slab choice, against 5.0e-4 for a float-floor FMA one.
+    X, Y, T, ndf, Z, W, _, (mn, mx), ps = case
+    tol = 1e-5 if X <= 256 else 2.5e-4
+    plan, sino, angles = sinogram_plan(case, storage)
+    d_in = C.DeviceArray.from_host(sino)
+    plan.filter(d_in)
+    d_vol = C.DeviceArray((Y, Z, W))
+    for k in ("direct", "staged"):
+        plan.set_backprojector(k)
+        plan.backproject(d_vol)
+        C.call("gx_device_synchronize")
+        vol = d_vol.to_host()
+        nrm = numpy.abs(sino).max(axis=(0, 1, 3)).astype(numpy.float32)
+        ys = [0, Y // 2, Y - 1]
+        rows = slab_rows_16(plan, case, ys, storage)
+        for y in ys:
+            s = (rows[y] * nrm[None, :, None]).astype(numpy.float32)
+            expect = O.project(s, (Z, W), angles, X / 2.0 - 0.7, ps, mn, mx)
+            assert rel_l2(vol[y], expect) < tol, (k, y)
+    plan.destroy()
+
+
 # ------------------------------------------------------------- autotuning
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
