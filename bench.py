@@ -569,6 +569,8 @@ def main():
                     help="requests of the served path (the workload n times "
                          "through the server; 0: skip)")
     ap.add_argument("--served-inflight", type=int, default=2)
+    ap.add_argument("--serve-inflight", type=int, default=4,
+                    help="requests in flight in the serving benchmark")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-host", action="store_true",
@@ -777,7 +779,8 @@ def main():
               file=sys.stderr, flush=True)
     n_serve = 64 if args.serve_requests is None else args.serve_requests
     if n_serve > 0 and args.only is None:
-        line["serving"] = serving_bench(C, n_serve, stream)
+        line["serving"] = serving_bench(C, n_serve, stream,
+                                        max_inflight=args.serve_inflight)
         print("bench: serving %s" % {k: v for k, v in line["serving"].items()
                                      if k != "description"},
               file=sys.stderr, flush=True)
@@ -812,7 +815,7 @@ def main_serving(args):
         sys.exit(1)
     stream = C.Stream()
     n = 256 if args.serve_requests is None else args.serve_requests
-    sv = serving_bench(C, n, stream)
+    sv = serving_bench(C, n, stream, max_inflight=args.serve_inflight)
     print(json.dumps({
         "metric": METRIC,
         "value": sv["updates_per_s"],
