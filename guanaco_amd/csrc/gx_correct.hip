@@ -225,22 +225,22 @@ hipError_t launch_ctf_grid(float2 *out, int w, int h, double ps,
 
 // ------------------------------------------------- pass 1: rows, R2C -------
 
-// Four image rows per workgroup (two complex pairs). Writes the half
-// spectrum F[t][cb][y][ci]; for each cb the four rows form one 128-byte
-// segment. Optionally accumulates max |image| per tilt (for reduced-precision
-// storage) with one atomic per workgroup.
-template <int X>
-__global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
+// 2 * RP image rows per workgroup (RP complex pairs). Writes the half
+// spectrum F[t][cb][y][ci]; for each cb the 2 * RP rows form one contiguous
+// run of 2 * RP * 32 bytes. Optionally accumulates max |image| per tilt (for
+// reduced-precision storage) with one atomic per workgroup.
+template <int X, int RP>
+__global__ __launch_bounds__(RP * (X / default_e(X))) void k_rows_r2c(
     const float *__restrict__ img, size_t tstride, int Y, int XB,
     float2 *__restrict__ F, uint32_t *__restrict__ tmax, int t0,
     const float2 *__restrict__ tw) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
-  __shared__ float2 s[2][fft_lds_len(X, E)];
+  __shared__ float2 s[RP][fft_lds_len(X, E)];
   __shared__ float red[16];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int t = blockIdx.y;
-  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const int ya = blockIdx.x * (2 * RP) + 2 * p, yb = ya + 1;
   const float *im = img + (size_t)t * tstride;
   float2 v[E];
   float amax = 0.0f;
@@ -259,13 +259,13 @@ __global__ __launch_bounds__(2 * (X / default_e(X))) void k_rows_r2c(
   for (int m = 0; m < E; ++m) s[p][tid + m * NT] = v[m];
   lds_barrier();
   if (tmax != nullptr) {
-    amax = block_max<2 * NT>(amax, red);
+    amax = block_max<RP * NT>(amax, red);
     if (threadIdx.x == 0) atomicMax(&tmax[t0 + t], __float_as_uint(amax));
   }
   const int ncol = XB * CB;
-  for (int e = threadIdx.x; e < 4 * ncol; e += 2 * NT) {
-    const int ci = e % CB, r = (e / CB) % 4, cb = e / (4 * CB);
-    const int kx = cb * CB + ci, y = blockIdx.x * 4 + r;
+  for (int e = threadIdx.x; e < 2 * RP * ncol; e += RP * NT) {
+    const int ci = e % CB, r = (e / CB) % (2 * RP), cb = e / (2 * RP * CB);
+    const int kx = cb * CB + ci, y = blockIdx.x * (2 * RP) + r;
     if (y >= Y) continue;
     float2 val = make_float2(0.0f, 0.0f);
     if (kx <= X / 2) {
@@ -1248,13 +1248,18 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  dim3 grid((Y + 3) / 4, ntilts);
+  // more row pairs per workgroup at 512 to 2048 points (each column block
+  // then written as 256 or 512 contiguous bytes), two elsewhere
   switch (X) {
 #define C_(n)                                                                  \
-  case n:                                                                      \
-    hipLaunchKernelGGL(k_rows_r2c<n>, grid, dim3(2 * (n / default_e(n))), 0, s, \
-                       img, tstride, Y, XB, F, tmax, t0, tw);                  \
-    break;
+  case n: {                                                                    \
+    constexpr int rp = n == 1024 ? 8 : (n == 512 || n == 2048) ? 4 : 2;        \
+    const dim3 grid((Y + 2 * rp - 1) / (2 * rp), ntilts);                      \
+    hipLaunchKernelGGL((k_rows_r2c<n, rp>), grid,                              \
+                       dim3(rp * (n / default_e(n))), 0, s, img, tstride, Y,   \
+                       XB, F, tmax, t0, tw);                                   \
+    break;                                                                     \
+  }
     GX_POW2_CASES(C_)
 #undef C_
     default:
