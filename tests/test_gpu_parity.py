@@ -232,6 +232,10 @@ RAMP_CTF_CASES = [
     (4096, 3, 3, 1, 4, 1.0, 25000.0),
     # 4096-point columns (2-column workgroups) under short rows
     (64, 4096, 2, 2, 4, 1.0, 25000.0),
+    # row counts that end inside a forward-transform workgroup (16 rows per
+    # workgroup at 1024 points, 8 at 2048)
+    (1024, 21, 3, 2, 4, 2.0, 22000.0),
+    (2048, 11, 2, 1, 4, 1.5, 28000.0),
 ]
 
 
