@@ -476,26 +476,36 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   const size_t kstride = (size_t)T * tstride;
   const ST *yblock = slab + (size_t)byb * row_len;
 
+  // The staging runs of a wave are wave-uniform: with the wave index taken
+  // from the first lane, each run's window entry is a scalar load and its
+  // buffer descriptor lives in scalar registers.
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
   uint4 pre[S::kSPW][S::kCPL];
   auto load_batch = [&](int tb) {
 #pragma unroll
     for (int i = 0; i < S::kSPW; ++i) {
-      const int q = wave + (kTX * kTZ / 64) * i;
+      const int q = wave_u + (kTX * kTZ / 64) * i;
       const int j = q / KS, sl = q % KS;
       const int t = tb + j;
       const int2 w = kload(tw, t);
       const int k = MULTI ? w.y + sl : 0;
       const bool ok = t < T && k < ndf;
       const ST *src = yblock + (size_t)t * tstride + (size_t)k * kstride;
+      // One range-checked raw buffer per run: its N positions of YB values.
+      // A load at an offset outside [0, N * YB * sizeof(T)) returns zeros
+      // and touches no memory, so positions outside [0, N), the chunks past
+      // the window (offset -1) and runs past the last tilt or slab (no
+      // records) all read as zero without branches.
+      const auto rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(ok ? src : yblock), (short)0,
+          ok ? (int)(row_len * sizeof(ST)) : 0, 0x00020000);
 #pragma unroll
       for (int c = 0; c < S::kCPL; ++c) {
         const int ch = lane + 64 * c;
-        const int pos = w.x + ch / S::kPosWords;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (ok && ch < S::kChunks && (unsigned)pos < (unsigned)N)
-          v = *(const uint4 *)(src + ((long)w.x * YB +
-                                      (long)ch * (16 / (int)sizeof(ST))));
-        pre[i][c] = v;
+        const int off = (w.x * YB + ch * (16 / (int)sizeof(ST))) * (int)sizeof(ST);
+        pre[i][c] = __builtin_bit_cast(
+            uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                       rs, ch < S::kChunks ? off : -1, 0, 0));
       }
     }
   };
