@@ -68,7 +68,7 @@ class VolumeDesc(ctypes.Structure):
         ("Ca", ctypes.c_double), ("Pa", ctypes.c_double),
         ("phi", ctypes.c_double),
         ("tilt_chunk", _sz), ("gpu_index", _int),
-        ("centres", _f32p),
+        ("centres", _f32p), ("band_rows", _sz),
     ]
 
 
@@ -177,6 +177,9 @@ SIGNATURES = {
     "gx_volume_filter": [_vp, _vp, _vp],
     "gx_volume_backproject": [_vp, _vp, _vp],
     "gx_volume_backproject_rows": [_vp, _sz, _sz, _vp, _vp],
+    "gx_volume_filter_rows": [_vp, _vp, _sz, _sz, _vp],
+    "gx_volume_plan_band": [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_sz),
+                            ctypes.POINTER(_sz)],
     "gx_volume_rows_per_block": [_vp, ctypes.POINTER(_sz)],
     "gx_volume_plan_set_backprojector": [_vp, _int],
     "gx_volume_plan_get_backprojector": [_vp, ctypes.POINTER(_int),
@@ -388,8 +391,11 @@ def volume_desc(num_pixels, num_rows, angles, depth, width=None,
                 num_defocus=1, pixel_size=1.0, centre=None, min_defocus=0.0,
                 max_defocus=0.0, weights=None, storage="f32",
                 input_layout=INPUT_PROJECTIONS, defocus=None, wavelength=0.0,
-                Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0, tilt_chunk=0, gpu_index=-1):
-    """A gx_volume_desc and the arrays it points to (keep them alive)."""
+                Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0, tilt_chunk=0, gpu_index=-1,
+                band_rows=0):
+    """A gx_volume_desc and the arrays it points to (keep them alive).
+    band_rows > 0: the plan holds the filtered slabs of that many detector
+    rows at a time (rounded up to a multiple of 32) and runs band by band."""
     angles = numpy.ascontiguousarray(angles, dtype=numpy.float32)
     T = angles.size
     weights = None if weights is None else numpy.ascontiguousarray(
@@ -411,7 +417,8 @@ def volume_desc(num_pixels, num_rows, angles, depth, width=None,
         STORAGE[storage] if isinstance(storage, str) else storage,
         input_layout, None if defocus is None else f64p(defocus),
         float(wavelength), float(Cs), float(Ca), float(Pa), float(phi),
-        tilt_chunk, gpu_index, None if centres is None else f32p(centres))
+        tilt_chunk, gpu_index, None if centres is None else f32p(centres),
+        int(band_rows))
     return desc, (angles, weights, defocus, centres)
 
 
@@ -423,12 +430,12 @@ class VolumePlan:
                  max_defocus=0.0, weights=None, storage="f32",
                  input_layout=INPUT_PROJECTIONS, defocus=None,
                  wavelength=0.0, Cs=0.0, Ca=0.0, Pa=0.0, phi=0.0,
-                 tilt_chunk=0, gpu_index=-1):
+                 tilt_chunk=0, gpu_index=-1, band_rows=0):
         self.desc, self._keep = volume_desc(
             num_pixels, num_rows, angles, depth, width, num_defocus,
             pixel_size, centre, min_defocus, max_defocus, weights, storage,
             input_layout, defocus, wavelength, Cs, Ca, Pa, phi, tilt_chunk,
-            gpu_index)
+            gpu_index, band_rows)
         width = num_pixels if width is None else width
         self.shape = (num_rows, depth, width)
         self.storage = storage
@@ -452,6 +459,25 @@ class VolumePlan:
     def backproject(self, d_vol, stream=None):
         call("gx_volume_backproject", self.handle, d_vol.ptr,
              stream.handle if isinstance(stream, Stream) else stream)
+
+    def filter_rows(self, d_in, y0, y1, stream=None):
+        """Filter detector rows [y0, y1) into the slabs (one band)."""
+        call("gx_volume_filter_rows", self.handle, d_in.ptr, int(y0), int(y1),
+             stream.handle if isinstance(stream, Stream) else stream)
+
+    def backproject_rows(self, y0, y1, d_vol, stream=None):
+        """Backproject rows [y0, y1) of the filtered band into d_vol,
+        which holds those rows."""
+        call("gx_volume_backproject_rows", self.handle, int(y0), int(y1),
+             d_vol.ptr, stream.handle if isinstance(stream, Stream)
+             else stream)
+
+    def band(self):
+        """(rows held per band, first and end row of the band held)."""
+        b, y0, y1 = _sz(), _sz(), _sz()
+        call("gx_volume_plan_band", self.handle, ctypes.byref(b),
+             ctypes.byref(y0), ctypes.byref(y1))
+        return b.value, y0.value, y1.value
 
     def set_backprojector(self, kind):
         """BP_DIRECT or BP_STAGED (the default where a tile spans <= 4
@@ -566,7 +592,8 @@ def plan_bytes(**kw):
         STORAGE[storage] if isinstance(storage, str) else storage,
         kw.get("input_layout", INPUT_PROJECTIONS),
         None if defocus is None else f64p(defocus),
-        0.0, 0.0, 0.0, 0.0, 0.0, kw.get("tilt_chunk", 0), -1, None)
+        0.0, 0.0, 0.0, 0.0, 0.0, kw.get("tilt_chunk", 0), -1, None,
+        int(kw.get("band_rows", 0)))
     n = _sz()
     call("gx_volume_plan_bytes", ctypes.byref(d), ctypes.byref(n))
     return n.value

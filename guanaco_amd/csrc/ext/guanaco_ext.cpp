@@ -261,10 +261,29 @@ void reconstruct_volume(
                         layout, centre, pixel_size, min_defocus, max_defocus,
                         weights, storage, defocus, wavelength, Cs, Ca, Pa, phi,
                         gpu_index);
+  // the caller's input holds exactly what the request reads
+  GUANACO_ASSERT((std::size_t)input.nbytes() ==
+                 guanaco::volume_input_bytes(r));
   const float *in = input.data();
   float *out = volume.mutable_data();
   py::gil_scoped_release release;
   guanaco::reconstruct_volume(r, in, out);
+}
+
+// Host bytes reconstruct_volume reads for these sizes (no device work).
+std::size_t volume_input_bytes(py::ssize_t num_pixels, py::ssize_t num_rows,
+                               py::ssize_t num_angles, py::ssize_t num_defocus,
+                               int layout, bool device_ctf) {
+  guanaco::VolumeRequest r;
+  r.num_pixels = num_pixels;
+  r.num_rows = num_rows;
+  r.num_angles = num_angles;
+  r.num_defocus = num_defocus;
+  r.depth = r.width = 1;
+  r.angles.assign(num_angles, 0.0f);
+  r.layout = (guanaco::Layout)layout;
+  if (device_ctf) r.defocus.assign(num_defocus, 0.0);
+  return guanaco::volume_input_bytes(r);
 }
 
 }  // namespace
@@ -365,6 +384,9 @@ PYBIND11_MODULE(guanaco_ext, m) {
         py::arg("defocus") = py::none(), py::arg("wavelength") = 0.0,
         py::arg("Cs") = 0.0, py::arg("Ca") = 0.0, py::arg("Pa") = 0.0,
         py::arg("phi") = 0.0, py::arg("gpu_index") = -1);
+  m.def("volume_input_bytes", &volume_input_bytes, py::arg("num_pixels"),
+        py::arg("num_rows"), py::arg("num_angles"), py::arg("num_defocus"),
+        py::arg("layout"), py::arg("device_ctf"));
 
   py::register_exception<guanaco::Error>(m, "Error", PyExc_RuntimeError);
 }

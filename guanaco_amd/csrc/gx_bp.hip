@@ -327,6 +327,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
     const typename StoreT<STORE>::T *__restrict__ slab,
     const float4 *__restrict__ coef, float dscale, float doffset, int ndf,
     int N, int T, int NYB, int W, int Z, int Y, int yb0, int y_base,
+    int slab_yb0,
     float *__restrict__ vol, int nbx, int nbz, unsigned nblocks) {
   constexpr bool kExact = STORE == kF32;
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp(
 
   const size_t row_len = (size_t)N * YB;
   const size_t kstride = (size_t)T * NYB * row_len;
-  const typename StoreT<STORE>::T *yblock = slab + (size_t)byb * row_len;
+  const typename StoreT<STORE>::T *yblock = slab + (size_t)(byb - slab_yb0) * row_len;
 
   for (int t = 0; t < T; ++t) {
     const float4 cf = kload(coef, t);
@@ -378,7 +379,8 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_rows(
     const typename StoreT<STORE>::T *__restrict__ slab,
     const float4 *__restrict__ coef, const float *__restrict__ row_off,
     int Y_tab, float dscale, float doffset, int ndf, int N, int T, int NYB,
-    int W, int Z, int Y, int yb0, int y_base, float *__restrict__ vol,
+    int W, int Z, int Y, int yb0, int y_base, int slab_yb0,
+    float *__restrict__ vol,
     int nbx, int nbz, unsigned nblocks) {
   constexpr bool kExact = STORE == kF32;
   using ST = StoreT<STORE>;
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_rows(
 
   const size_t row_len = (size_t)N * YB;
   const size_t kstride = (size_t)T * NYB * row_len;
-  const typename ST::T *yblock = slab + (size_t)byb * row_len;
+  const typename ST::T *yblock = slab + (size_t)(byb - slab_yb0) * row_len;
 
   for (int t = 0; t < T; ++t) {
     const float4 cf = kload(coef, t);
@@ -451,7 +453,8 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
     const typename StoreT<STORE>::T *__restrict__ slab,
     const float4 *__restrict__ coef, const int2 *__restrict__ win,
     float dscale, float doffset, int ndf, int N, int T, int NYB, int W, int Z,
-    int Y, int yb0, int y_base, float *__restrict__ vol, int nbx, int nbz,
+    int Y, int yb0, int y_base, int slab_yb0, float *__restrict__ vol,
+    int nbx, int nbz,
     unsigned nblocks) {
   using S = Staging<YB, STORE, KS>;
   using ST = typename StoreT<STORE>::T;
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   const size_t row_len = (size_t)N * YB;
   const size_t tstride = (size_t)NYB * row_len;
   const size_t kstride = (size_t)T * tstride;
-  const ST *yblock = slab + (size_t)byb * row_len;
+  const ST *yblock = slab + (size_t)(byb - slab_yb0) * row_len;
 
   // The staging runs of a wave are wave-uniform: with the wave index taken
   // from the first lane, each run's window entry is a scalar load and its
@@ -586,7 +589,7 @@ static hipError_t launch_bp_lds_k(const BpArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_bp_lds<YB, STORE, MULTI, KS>), dim3(nb),
                      dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef, a.win,
                      a.dscale, a.doffset, a.ndf, a.N, a.T, a.NYB, a.W, a.Z,
-                     a.Y, a.yb0, a.y_base, a.vol, nbx, nbz, nb);
+                     a.Y, a.yb0, a.y_base, a.slab_yb0, a.vol, nbx, nbz, nb);
   return hipGetLastError();
 }
 
@@ -611,12 +614,12 @@ static hipError_t launch_bp_t(const BpArgs &a, hipStream_t s) {
   if (a.ndf > 1) {
     hipLaunchKernelGGL((k_bp<YB, STORE, true>), dim3(nb), dim3(kTX * kTZ), 0,
                        s, (const ST *)a.slab, a.coef, a.dscale, a.doffset,
-                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base,
+                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.slab_yb0,
                        a.vol, nbx, nbz, nb);
   } else {
     hipLaunchKernelGGL((k_bp<YB, STORE, false>), dim3(nb), dim3(kTX * kTZ), 0,
                        s, (const ST *)a.slab, a.coef, a.dscale, a.doffset,
-                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base,
+                       a.ndf, a.N, a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.slab_yb0,
                        a.vol, nbx, nbz, nb);
   }
   return hipGetLastError();
@@ -634,13 +637,13 @@ static hipError_t launch_bp_rows_t(const BpArgs &a, hipStream_t s) {
     hipLaunchKernelGGL((k_bp_rows<YB, STORE, true>), dim3(nb),
                        dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef,
                        a.row_off, a.Y_tab, a.dscale, a.doffset, a.ndf, a.N,
-                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.vol,
+                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.slab_yb0, a.vol,
                        nbx, nbz, nb);
   } else {
     hipLaunchKernelGGL((k_bp_rows<YB, STORE, false>), dim3(nb),
                        dim3(kTX * kTZ), 0, s, (const ST *)a.slab, a.coef,
                        a.row_off, a.Y_tab, a.dscale, a.doffset, a.ndf, a.N,
-                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.vol,
+                       a.T, a.NYB, a.W, a.Z, a.Y, a.yb0, a.y_base, a.slab_yb0, a.vol,
                        nbx, nbz, nb);
   }
   return hipGetLastError();

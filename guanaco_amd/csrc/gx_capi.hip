@@ -88,6 +88,11 @@ int select_device(int gpu_index) {
   return GX_OK;
 }
 
+}  // namespace
+
+// Host tables shared with the library's other translation units.
+namespace gx {
+
 // reconstructor.cpp:29-47 in the reference's float/double arithmetic.
 void tilt_geometry(size_t T, const float *angles, float centre, float *coef) {
   const double output_scale = M_PI / (double)(2 * T);
@@ -171,6 +176,15 @@ size_t filter_length(size_t N) {
   return L < 8 ? 8 : L;
 }
 
+}  // namespace gx
+
+namespace {
+
+using gx::defocus_coefficients;
+using gx::filter_length;
+using gx::ramp_table;
+using gx::tilt_geometry;
+
 // The power-of-two correction kernels take 8 <= X <= 4096 and 4 <= Y <=
 // 4096; other sizes use Bluestein transforms of length filter_length(n).
 bool rows_pow2(size_t X) { return is_pow2(X) && X >= 8 && X <= 4096; }
@@ -240,6 +254,10 @@ gx::BluesteinTable bluestein_view(const float2 *d, size_t n, size_t M) {
 
 }  // namespace
 
+// The twiddle table of every transform, one copy per GPU in a device
+// global: filled on the first use on a device, never allocated or freed.
+static __device__ __align__(256) float2 twiddle_table[gx::tw_table_len()];
+
 namespace gx {
 
 const float2 *twiddles(hipError_t *err) {
@@ -265,16 +283,15 @@ const float2 *twiddles(hipError_t *err) {
         if (tw_count(1 << ln, e) > 0)
           tw_fill_block(1 << ln, e, h.data(),
                         h.data() + kTwN + tw_block_offset(1 << ln, e));
-    float2 *d = nullptr;
-    *err = hipMalloc(&d, sizeof(float2) * h.size());
+    // the device address of this GPU's copy of the array (its name is only
+    // a host-side stand-in)
+    void *d = nullptr;
+    *err = hipGetSymbolAddress(&d, HIP_SYMBOL(twiddle_table));
     if (*err != hipSuccess) return nullptr;
     *err = hipMemcpy(d, h.data(), sizeof(float2) * h.size(),
                      hipMemcpyHostToDevice);
-    if (*err != hipSuccess) {
-      (void)hipFree(d);
-      return nullptr;
-    }
-    tables[dev] = d;  // held for the life of the process
+    if (*err != hipSuccess) return nullptr;
+    tables[dev] = (float2 *)d;
   }
   *err = hipSuccess;
   return tables[dev];
@@ -460,7 +477,12 @@ extern "C" int gx_graph_destroy(void *graph_exec) {
 
 // ------------------------------------ reference-equivalent entry points ---
 
-static int check_recon_cfg(const gx_recon_cfg *c) {
+namespace gx {
+// config.h:66-75, and the filter's size limit
+int check_recon_cfg(const gx_recon_cfg *c);
+}  // namespace gx
+
+int gx::check_recon_cfg(const gx_recon_cfg *c) {
   GX_CHECK(c != nullptr);
   // config.h:66-75
   GX_CHECK(c->num_pixels > 0);
@@ -705,7 +727,11 @@ struct gx_volume_plan {
   int device = 0;
   int X = 0, Y = 0, T = 0, ndf = 0, Z = 0, W = 0;
   int storage = 0, layout = 0;
+  // NYB: row blocks of the slab buffer, which holds `band` detector rows
+  // (Y unless the plan is banded): rows [slab_y0, slab_y1) after the last
+  // filter pass
   int YB = 0, NYB = 0, XB = 0, L = 0, chunk = 0;
+  int band = 0, slab_y0 = 0, slab_y1 = 0;
   int bp_kind = GX_BP_STAGED;
   int bp_staged = 1;  // slabs per tilt of the staged kernel; 0 = direct
   bool ctf = false;
@@ -791,7 +817,7 @@ bool tile_valid(const gx_volume_plan *p, int rows, int kind) {
 
 void set_tile(gx_volume_plan *p, int rows, int kind) {
   p->YB = rows;
-  p->NYB = (p->Y + rows - 1) / rows;
+  p->NYB = (p->band + rows - 1) / rows;
   p->bp_kind = kind;
 }
 
@@ -819,7 +845,7 @@ constexpr size_t kPlanAlign = 256;
 constexpr int kPlanBuffers = 13;
 
 struct PlanSizes {
-  int YB, NYB, XB, L, chunk;
+  int YB, NYB, XB, L, chunk, band;
   bool ctf;
   size_t slab_bytes, mask_bytes, f_bytes, g_bytes, small_bytes, total;
 };
@@ -855,7 +881,15 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
     return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", X);
   }
   ps->YB = d->storage == GX_STORE_F32 ? 16 : 32;
-  ps->NYB = (int)((Y + ps->YB - 1) / ps->YB);
+  // A banded plan holds the filtered slabs of band_rows detector rows at a
+  // time (a multiple of 32, so of every tile's row count); 0: every row.
+  size_t band = Y;
+  if (d->band_rows > 0 && d->band_rows < Y) {
+    band = (d->band_rows + 31) / 32 * 32;
+    if (band > Y) band = Y;
+  }
+  ps->band = (int)band;
+  ps->NYB = (int)((band + ps->YB - 1) / ps->YB);
   ps->XB = (int)(X / 2 / CB + 1);
   ps->L = (int)(ps->ctf && rows_pow2(X) ? 2 * X : filter_length(X));
   const size_t es = d->storage == GX_STORE_F32 ? 4 : 2;
@@ -918,6 +952,9 @@ void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
   p->layout = d->input_layout;
   p->YB = ps.YB;
   p->NYB = ps.NYB;
+  p->band = ps.band;
+  p->slab_y0 = 0;
+  p->slab_y1 = ps.band == p->Y ? p->Y : 0;
   p->XB = ps.XB;
   p->L = ps.L;
   p->chunk = ps.chunk;
@@ -1105,19 +1142,25 @@ extern "C" int gx_volume_plan_destroy(gx_volume_plan *plan) {
   return GX_OK;
 }
 
-extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
-                                void *stream) {
+extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
+                                     size_t y0, size_t y1, void *stream) {
   GX_CHECK(p != nullptr && d_in != nullptr);
+  GX_CHECK(y0 < y1 && y1 <= (size_t)p->Y);
+  GX_CHECK(y0 % (size_t)p->YB == 0);
+  GX_CHECK(y1 - y0 <= (size_t)p->band);
   hipStream_t s = (hipStream_t)stream;
   const bool reduced = p->storage != GX_STORE_F32;
   uint32_t *tmax = reduced ? p->tmax.as<uint32_t>() : nullptr;
   if (reduced) GX_HIP(launch_fill_u32(tmax, 0u, (size_t)p->T, s));
   const size_t X = p->X, Y = p->Y, T = p->T, ndf = p->ndf;
+  const int by0 = (int)y0, bny = (int)(y1 - y0);
   const gx::BluesteinTable bx =
       bluestein_view(p->bs.as<float2>(), (size_t)p->X, (size_t)p->MX);
   const gx::BluesteinTable by = bluestein_view(
       p->bs.as<float2>() + p->bs_y, (size_t)p->Y, (size_t)p->MY);
   if (p->ctf) {
+    // The 2D correction needs whole projections: every row is transformed,
+    // the column pass writes and the row pass reads only the band's rows.
     for (int t0 = 0; t0 < p->T; t0 += p->chunk) {
       const int tc = p->T - t0 < p->chunk ? p->T - t0 : p->chunk;
       const float *img = d_in + (size_t)t0 * Y * X;
@@ -1131,21 +1174,22 @@ extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
       if (p->cols_bs) {
         GX_HIP(launch_cols_bs(p->F.as<float2>(), p->mask.as<int8_t>(),
                               p->G.as<float2>(), p->Y, p->XB, p->ndf, tc,
-                              p->MY, by, s));
+                              p->MY, by, s, by0, by0 + bny));
       } else {
         GX_HIP(launch_cols(p->F.as<float2>(), p->mask.as<int8_t>(),
-                           p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s));
+                           p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s,
+                           by0, by0 + bny));
       }
       if (p->rows_bs) {
         GX_HIP(launch_rows_c2r_filter_bs(
             p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
             p->ramp.as<float>(), p->wscale.as<float>(), tmax, p->nrm_mul,
-            p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s));
+            p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s, by0, bny));
       } else {
         GX_HIP(launch_rows_c2r_filter(
             p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
             p->wscale.as<float>(), tmax, p->nrm_mul,
-            p->storage, p->slab.p, p->NYB, p->YB, s));
+            p->storage, p->slab.p, p->NYB, p->YB, s, by0, bny));
       }
     }
   } else {
@@ -1159,15 +1203,30 @@ extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
       sk = Y * X;
       sy = X;
     }
+    // rows are independent: the band's rows are a view of the input
+    const float *src = d_in + y0 * sy;
     if (reduced)
-      GX_HIP(launch_absmax_rows(d_in, st, sk, sy, p->X, p->Y, p->T, p->ndf,
+      GX_HIP(launch_absmax_rows(src, st, sk, sy, p->X, bny, p->T, p->ndf,
                                 tmax, s));
-    GX_HIP(launch_filter_rows(d_in, st, sk, sy, p->X, p->L, p->Y, p->T, p->ndf,
+    GX_HIP(launch_filter_rows(src, st, sk, sy, p->X, p->L, bny, p->T, p->ndf,
                               p->ramp.as<float>(), p->wscale.as<float>(), tmax,
                               p->nrm_mul, p->storage, p->slab.p, p->NYB, p->YB,
                               s));
   }
+  p->slab_y0 = (int)y0;
+  p->slab_y1 = (int)y1;
   return GX_OK;
+}
+
+extern "C" int gx_volume_filter(gx_volume_plan *p, const float *d_in,
+                                void *stream) {
+  GX_CHECK(p != nullptr);
+  if (p->band < p->Y)
+    return fail(GX_EINVAL,
+                "the plan holds %d of %d rows at a time: filter bands with "
+                "gx_volume_filter_rows or run the whole path with "
+                "gx_volume_run", p->band, p->Y);
+  return gx_volume_filter_rows(p, d_in, 0, (size_t)p->Y, stream);
 }
 
 extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
@@ -1177,6 +1236,10 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   GX_CHECK(y0 <= y1 && y1 <= (size_t)p->Y);
   GX_CHECK(y0 % (size_t)p->YB == 0);
   GX_CHECK(y1 % (size_t)p->YB == 0 || y1 == (size_t)p->Y);
+  if (y0 < y1 && (y0 < (size_t)p->slab_y0 || y1 > (size_t)p->slab_y1))
+    return fail(GX_EINVAL,
+                "rows [%zu, %zu) are not in the filtered band [%d, %d)", y0,
+                y1, p->slab_y0, p->slab_y1);
   hipStream_t s = (hipStream_t)stream;
   BpArgs a{};
   a.slab = p->slab.p;
@@ -1200,6 +1263,7 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
   a.yb0 = (int)(y0 / p->YB);
   a.yb1 = (int)((y1 + p->YB - 1) / p->YB);
   a.y_base = (int)y0;
+  a.slab_yb0 = p->slab_y0 / p->YB;
   a.vol = d_vol;
   a.staged = p->bp_kind == GX_BP_STAGED ? p->bp_staged : 0;
   a.row_off = p->per_row ? p->row_off.as<float>() : nullptr;
@@ -1211,6 +1275,11 @@ extern "C" int gx_volume_backproject_rows(gx_volume_plan *p, size_t y0,
 extern "C" int gx_volume_backproject(gx_volume_plan *p, float *d_vol,
                                      void *stream) {
   GX_CHECK(p != nullptr);
+  if (p->band < p->Y)
+    return fail(GX_EINVAL,
+                "the plan holds %d of %d rows at a time: backproject bands "
+                "with gx_volume_backproject_rows or run the whole path with "
+                "gx_volume_run", p->band, p->Y);
   return gx_volume_backproject_rows(p, 0, (size_t)p->Y, d_vol, stream);
 }
 
@@ -1244,9 +1313,27 @@ extern "C" int gx_volume_rows_per_block(gx_volume_plan *p, size_t *rows) {
 
 extern "C" int gx_volume_run(gx_volume_plan *p, const float *d_in,
                              float *d_vol, void *stream) {
-  int rc = gx_volume_filter(p, d_in, stream);
-  if (rc) return rc;
-  return gx_volume_backproject(p, d_vol, stream);
+  GX_CHECK(p != nullptr && d_vol != nullptr);
+  // one band of rows at a time: filter it into the slabs, backproject it
+  // into its rows of the volume
+  for (size_t y0 = 0; y0 < (size_t)p->Y; y0 += (size_t)p->band) {
+    const size_t y1 = std::min((size_t)p->Y, y0 + (size_t)p->band);
+    int rc = gx_volume_filter_rows(p, d_in, y0, y1, stream);
+    if (rc) return rc;
+    rc = gx_volume_backproject_rows(p, y0, y1,
+                                    d_vol + y0 * (size_t)p->Z * p->W, stream);
+    if (rc) return rc;
+  }
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_band(gx_volume_plan *p, size_t *band_rows,
+                                   size_t *y0, size_t *y1) {
+  GX_CHECK(p != nullptr);
+  if (band_rows) *band_rows = (size_t)p->band;
+  if (y0) *y0 = (size_t)p->slab_y0;
+  if (y1) *y1 = (size_t)p->slab_y1;
+  return GX_OK;
 }
 
 extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
@@ -1292,12 +1379,14 @@ extern "C" int gx_volume_plan_autotune(gx_volume_plan *p, const float *d_in,
   const int kinds[2] = {GX_BP_STAGED, GX_BP_DIRECT};
   float best = 0.0f;
   int rc;
+  // a banded plan is timed on its first band (rows [0, band))
+  const size_t yt = (size_t)p->band;
   for (int ri = 0; ri < 2; ++ri) {
     set_tile(p, rows[ri], p->bp_kind);
     // the first pass warms up; the second is timed
-    if ((rc = gx_volume_filter(p, d_in, s))) return rc;
+    if ((rc = gx_volume_filter_rows(p, d_in, 0, yt, s))) return rc;
     GX_HIP(hipEventRecord(ev.a, s));
-    if ((rc = gx_volume_filter(p, d_in, s))) return rc;
+    if ((rc = gx_volume_filter_rows(p, d_in, 0, yt, s))) return rc;
     GX_HIP(hipEventRecord(ev.b, s));
     GX_HIP(hipEventSynchronize(ev.b));
     float fms = 0.0f;
@@ -1305,10 +1394,10 @@ extern "C" int gx_volume_plan_autotune(gx_volume_plan *p, const float *d_in,
     for (int kind : kinds) {
       if (!tile_valid(p, rows[ri], kind)) continue;
       p->bp_kind = kind;
-      if ((rc = gx_volume_backproject(p, d_vol, s))) return rc;
+      if ((rc = gx_volume_backproject_rows(p, 0, yt, d_vol, s))) return rc;
       GX_HIP(hipEventRecord(ev.a, s));
       for (int i = 0; i < 2; ++i)
-        if ((rc = gx_volume_backproject(p, d_vol, s))) return rc;
+        if ((rc = gx_volume_backproject_rows(p, 0, yt, d_vol, s))) return rc;
       GX_HIP(hipEventRecord(ev.b, s));
       GX_HIP(hipEventSynchronize(ev.b));
       float bms = 0.0f;

@@ -305,7 +305,8 @@ template <int Y, typename M, int CW>
 __global__ __launch_bounds__(CW *(Y / default_e(Y)))
 __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
     const float2 *__restrict__ F, const M *__restrict__ mask,
-    float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw) {
+    float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw,
+    int ylo, int yhi) {
   constexpr int E = default_e(Y);
   constexpr int NT = Y / E;
   constexpr int H = CB / CW;  // workgroups per column block
@@ -362,9 +363,14 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
       }
     }
     Fft<Y, E, 1, CW>::run(w, s + cl, tid, twy);
+    // only the rows [ylo, yhi) that the row pass reads (a band of a
+    // banded plan; every row otherwise)
     float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
 #pragma unroll
-    for (int m = 0; m < E; ++m) dst[(size_t)(tid + m * NT) * CB + ci] = w[m];
+    for (int m = 0; m < E; ++m) {
+      const int y = tid + m * NT;
+      if (y >= ylo && y < yhi) dst[(size_t)y * CB + ci] = w[m];
+    }
   }
 }
 
@@ -440,7 +446,7 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
                       int t0, int T, const float *__restrict__ wscale,
                       const uint32_t *__restrict__ tmax, float nrm_mul,
                       typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-                      int YB, const float2 *__restrict__ tw) {
+                      int YB, const float2 *__restrict__ tw, int y0, int ny) {
   constexpr int L = 2 * X;
   constexpr int EL = default_e(L);
   constexpr int EX = EL / 2;
@@ -451,7 +457,8 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   __shared__ float2 s[2][fft_lds_len(L, EL)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
-  const int r0 = blockIdx.x * (4 * IT) + p * (2 * IT);  // first row of slot
+  // first row of the slot within the band [y0, y0 + ny) the slab holds
+  const int r0 = blockIdx.x * (4 * IT) + p * (2 * IT);
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
 
   // the L-point twiddles serve both L-point transforms of every row pair
@@ -466,7 +473,7 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   PT out[EL / 2][IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    const int ya = r0 + 2 * it, yb = ya + 1;
+    const int ya = y0 + r0 + 2 * it, yb = ya + 1;
     float2 v[EX];
 #pragma unroll
     for (int m = 0; m < EX; ++m)
@@ -493,8 +500,8 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
       out[m][it] = RowPair<STORE>::pack(u[m].x * sc, u[m].y * sc);
   }
   using ST = typename StoreT<STORE>::T;
-  if (r0 >= Y) return;
-  const bool whole = r0 + 2 * IT <= Y;
+  if (r0 >= ny) return;
+  const bool whole = r0 + 2 * IT <= ny;
 #pragma unroll
   for (int m = 0; m < EL / 2; ++m) {
     const int n = tid + m * NT;
@@ -516,7 +523,7 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
       const ST *src = reinterpret_cast<const ST *>(out[m]);
 #pragma unroll
       for (int r = 0; r < 2 * IT; ++r)  // constant indices keep out[] in VGPRs
-        if (r0 + r < Y) dst[r] = src[r];
+        if (r0 + r < ny) dst[r] = src[r];
     }
   }
 }
@@ -670,7 +677,7 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
     const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-    int YB, const float2 *__restrict__ tw) {
+    int YB, const float2 *__restrict__ tw, int y0, int ny) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
   constexpr bool WAVE = NT <= 64;
@@ -689,11 +696,14 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
   // takes consecutive row groups: the RB-row runs of one slab row block, and
   // the halves of the 128-byte spectrum lines that row groups share, then
   // meet in that XCD's L2 instead of reaching HBM as partial lines.
-  const unsigned nry = (unsigned)(Y + RB - 1) / RB;
+  // Row groups cover the band [y0, y0 + ny) the slab holds; R0 counts from
+  // y0, and the spectrum rows read are y0 + R0 onwards.
+  const unsigned nry = (unsigned)(ny + RB - 1) / RB;
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
   const int k = (int)((lb / nry) % (unsigned)ndf);
   const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
   const int R0 = (int)(lb % nry) * RB;
+  const int ya0 = y0 + R0;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
   float2 *ls = s + p * SL;
 
@@ -707,14 +717,15 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
 
   PT out[E];
   if constexpr (IT == 1) {
-    ramp_pair<X, STORE, WAVE>(g, Y, R0 + 2 * p, R0 + 2 * p + 1, ls, tid, twx,
-                              ct, tw, sc, out);
-    store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, Y, X, T,
+    ramp_pair<X, STORE, WAVE>(g, Y, ya0 + 2 * p, ya0 + 2 * p + 1, ls, tid,
+                              twx, ct, tw, sc, out);
+    store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, ny, X, T,
                                  NYB, YB);
   } else {
-    const int Rs = R0 + 4 * p;  // this slot's four rows
+    const int Rs = R0 + 4 * p;  // this slot's four rows (band-relative)
+    const int ys = y0 + Rs;
     PT *sp = stash + p * X;
-    ramp_pair<X, STORE, WAVE>(g, Y, Rs, Rs + 1, ls, tid, twx, ct, tw, sc, out);
+    ramp_pair<X, STORE, WAVE>(g, Y, ys, ys + 1, ls, tid, twx, ct, tw, sc, out);
 #pragma unroll
     for (int m = 0; m < E; ++m) sp[tid + m * NT] = out[m];
     // The second pair recomputes its per-column factors (c(n), ramp)
@@ -723,10 +734,10 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
     int tid2 = tid;
     float2 ct2 = ct;
     asm volatile("" : "+v"(tid2), "+v"(ct2)::"memory");
-    ramp_pair<X, STORE, WAVE>(g, Y, Rs + 2, Rs + 3, ls, tid2, twx, ct2, tw, sc,
+    ramp_pair<X, STORE, WAVE>(g, Y, ys + 2, ys + 3, ls, tid2, twx, ct2, tw, sc,
                               out);
-    if (Rs >= Y) return;
-    const bool whole = Rs + 4 <= Y;  // Rs % 4 == 0: the run is 8-byte aligned
+    if (Rs >= ny) return;
+    const bool whole = Rs + 4 <= ny;  // Rs % 4 == 0: the run is 8-byte aligned
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
@@ -738,7 +749,7 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
         const ST *v = reinterpret_cast<const ST *>(&w);
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (Rs + r < Y) dst[r] = v[r];
+          if (Rs + r < ny) dst[r] = v[r];
       }
     }
   }
@@ -1115,7 +1126,7 @@ template <int M, int CW>
 __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
     const float2 *__restrict__ F, const int8_t *__restrict__ mask,
     float2 *__restrict__ G, int Y, int XB, int ndf,
-    const float2 *__restrict__ tw, BluesteinTable by) {
+    const float2 *__restrict__ tw, BluesteinTable by, int ylo, int yhi) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
   static_assert(CB % CW == 0, "column groups split a block");
@@ -1148,7 +1159,7 @@ __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int y = tid + m * NT;
-      if (y < Y) dst[(size_t)y * CB + ci] = w[m];
+      if (y >= ylo && y < yhi) dst[(size_t)y * CB + ci] = w[m];
     }
   }
 }
@@ -1198,13 +1209,15 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void
                          const uint32_t *__restrict__ tmax, float nrm_mul,
                          typename StoreT<STORE>::T *__restrict__ slab,
                          int NYB, int YB, const float2 *__restrict__ tw,
-                         BluesteinTable bx) {
+                         BluesteinTable bx, int y0, int ny) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
   __shared__ float2 s[2][fft_lds_len(M, E)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
-  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  // band-relative rows ra, rb of the slab; spectrum rows y0 + ra, y0 + rb
+  const int ra = blockIdx.x * 4 + 2 * p, rb = ra + 1;
+  const int ya = y0 + ra, yb = ya + 1;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
   typename Fft<M, E, -1, 1>::Twiddles twm;
   twm.load(tid, tw);
@@ -1231,8 +1244,8 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void
   for (int m = 0; m < E; ++m) {
     const int n = tid + m * NT;
     if (n < X) {
-      if (ya < Y) slab[slab_index(k, t, ya, n, T, NYB, X, YB)] = ST::pack(v[m].x * sc);
-      if (yb < Y) slab[slab_index(k, t, yb, n, T, NYB, X, YB)] = ST::pack(v[m].y * sc);
+      if (ra < ny) slab[slab_index(k, t, ra, n, T, NYB, X, YB)] = ST::pack(v[m].x * sc);
+      if (rb < ny) slab[slab_index(k, t, rb, n, T, NYB, X, YB)] = ST::pack(v[m].y * sc);
     }
   }
 }
@@ -1271,7 +1284,8 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
 template <typename M>
 static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
                                 int Y, int XB, int ndf, int ntilts,
-                                hipStream_t s) {
+                                hipStream_t s, int ylo, int yhi) {
+  if (yhi < 0) yhi = Y;
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
@@ -1282,7 +1296,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   case n:                                                                  \
     hipLaunchKernelGGL((k_cols<n, M, CB>), grid,                           \
                        dim3(CB * (n / default_e(n))), 0, s, F, mask, G, XB, \
-                       ndf, tw);                                           \
+                       ndf, tw, ylo, yhi);                                 \
     break;
     C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
     C_(2048)
@@ -1291,7 +1305,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   case n:                                                                  \
     hipLaunchKernelGGL((k_cols<n, M, CB / 2>), grid2,                      \
                        dim3(CB / 2 * (n / default_e(n))), 0, s, F, mask, G, \
-                       XB, ndf, tw);                                       \
+                       XB, ndf, tw, ylo, yhi);                             \
     break;
     C_(4096)
 #undef C_
@@ -1302,14 +1316,15 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
 }
 
 hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
-                       int XB, int ndf, int ntilts, hipStream_t s) {
-  return launch_cols_t<int8_t>(F, mask, G, Y, XB, ndf, ntilts, s);
+                       int XB, int ndf, int ntilts, hipStream_t s, int ylo,
+                       int yhi) {
+  return launch_cols_t<int8_t>(F, mask, G, Y, XB, ndf, ntilts, s, ylo, yhi);
 }
 
 hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
                                 float2 *G, int Y, int XB, int ntilts,
                                 hipStream_t s) {
-  return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s);
+  return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s, 0, Y);
 }
 
 // Rows per workgroup: the slab's row blocking YB, at most 16 (float32) or
@@ -1327,16 +1342,17 @@ static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
                                        const float *wscale,
                                        const uint32_t *tmax, float nrm_mul,
                                        void *slab, int NYB, int YB,
-                                       hipStream_t s, const float2 *tw) {
+                                       hipStream_t s, const float2 *tw,
+                                       int y0, int ny) {
   using ST = typename StoreT<STORE>::T;
-  dim3 grid((Y + 4 * IT - 1) / (4 * IT), ndf, ntilts);
+  dim3 grid((ny + 4 * IT - 1) / (4 * IT), ndf, ntilts);
   switch (X) {
 #define C_(n)                                                                  \
   case n:                                                                      \
     hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE, IT>), grid,                \
                        dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
                        ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
-                       tw);                                                    \
+                       tw, y0, ny);                                            \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128)
 #undef C_
@@ -1360,11 +1376,11 @@ static void launch_c2r_ramp_x(dim3 grid, const float2 *G, int Y, int XB,
                               int ndf, int t0, int T, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, void *slab,
                               int NYB, int YB, hipStream_t s,
-                              const float2 *tw) {
+                              const float2 *tw, int y0, int ny) {
   using ST = typename StoreT<STORE>::T;
   hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P, IT>), grid,
                      dim3(P * (X / default_e(X))), 0, s, G, Y, XB, ndf, t0, T,
-                     wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw);
+                     wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw, y0, ny);
 }
 
 template <int STORE, int P>
@@ -1372,13 +1388,14 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     int ndf, int t0, int ntilts, int T,
                                     const float *wscale, const uint32_t *tmax,
                                     float nrm_mul, void *slab, int NYB, int YB,
-                                    hipStream_t s, const float2 *tw) {
+                                    hipStream_t s, const float2 *tw, int y0,
+                                    int ny) {
   // two row pairs per slot at 4096 points with 16-bit storage
   const int rb = X == 4096 && STORE != kF32 ? 4 * P : 2 * P;
-  const size_t nb = (size_t)((Y + rb - 1) / rb) * ndf * ntilts;
+  const size_t nb = (size_t)((ny + rb - 1) / rb) * ndf * ntilts;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
-#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny
   switch (X) {
     case 256: launch_c2r_ramp_x<256, STORE, P>(A_); break;
     case 512: launch_c2r_ramp_x<512, STORE, P>(A_); break;
@@ -1411,8 +1428,9 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       const float *wscale,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
-                                      hipStream_t s, const float2 *tw) {
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+                                      hipStream_t s, const float2 *tw, int y0,
+                                      int ny) {
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny
   if (X >= 256 && YB >= 8) {
     switch (c2r_ramp_slots(X, YB)) {
       case 1: return launch_c2r_ramp_p<STORE, 1>(A_);
@@ -1440,23 +1458,26 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
-                                  hipStream_t s) {
+                                  hipStream_t s, int y0, int ny) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
+  if (ny < 0) ny = Y - y0;
+  if (y0 < 0 || y0 % YB != 0 || ny <= 0 || y0 + ny > Y)
+    return hipErrorInvalidValue;
   switch (store) {
     case kF32:
       return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw);
+                                       s, tw, y0, ny);
     case kF16:
       return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw);
+                                       s, tw, y0, ny);
     case kBF16:
       return launch_c2r_filter_t<kBF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                         wscale, tmax, nrm_mul, slab,
-                                        NYB, YB, s, tw);
+                                        NYB, YB, s, tw, y0, ny);
   }
   return hipErrorInvalidValue;
 }
@@ -1696,7 +1717,9 @@ hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
 
 hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                           int Y, int XB, int ndf, int ntilts, int M,
-                          const BluesteinTable &by, hipStream_t s) {
+                          const BluesteinTable &by, hipStream_t s, int ylo,
+                          int yhi) {
+  if (yhi < 0 || yhi > Y) yhi = Y;
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
@@ -1707,7 +1730,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
   case n:                                                                     \
     hipLaunchKernelGGL((k_cols_bs<n, CB>), dim3(XB, ntilts),                  \
                        dim3(CB * (n / default_e(n))), 0, s, F, mask, G, Y,    \
-                       XB, ndf, tw, by);                                      \
+                       XB, ndf, tw, by, ylo, yhi);                            \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
     C_(4096)
@@ -1715,7 +1738,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
     case 8192:
       hipLaunchKernelGGL((k_cols_bs<8192, CB / 2>), dim3(XB * 2, ntilts),
                          dim3((CB / 2) * (8192 / default_e(8192))), 0, s, F,
-                         mask, G, Y, XB, ndf, tw, by);
+                         mask, G, Y, XB, ndf, tw, by, ylo, yhi);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1752,16 +1775,17 @@ static hipError_t launch_c2r_filter_bs_t(const float2 *G, int X, int Y,
                                          const uint32_t *tmax, float nrm_mul,
                                          void *slab, int NYB, int YB, int M,
                                          const BluesteinTable &bx,
-                                         hipStream_t s, const float2 *tw) {
+                                         hipStream_t s, const float2 *tw,
+                                         int y0, int ny) {
   using ST = typename StoreT<STORE>::T;
-  dim3 grid((Y + 3) / 4, ndf, ntilts);
+  dim3 grid((ny + 3) / 4, ndf, ntilts);
   switch (M) {
 #define C_(n)                                                                  \
   case n:                                                                      \
     hipLaunchKernelGGL((k_rows_c2r_filter_bs<n, STORE>), grid,                 \
                        dim3(2 * (n / default_e(n))), 0, s, G, X, Y, XB, ndf,   \
                        t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
-                       tw, bx);                                                \
+                       tw, bx, y0, ny);                                        \
     break;
     GX_BS_CASES(C_)
 #undef C_
@@ -1777,11 +1801,14 @@ hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
                                      const uint32_t *tmax, float nrm_mul,
                                      int store, void *slab, int NYB, int YB,
                                      int M, const BluesteinTable &bx,
-                                     hipStream_t s) {
+                                     hipStream_t s, int y0, int ny) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, M, bx, s, tw
+  if (ny < 0) ny = Y - y0;
+  if (y0 < 0 || y0 % YB != 0 || ny <= 0 || y0 + ny > Y)
+    return hipErrorInvalidValue;
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, M, bx, s, tw, y0, ny
   switch (store) {
     case kF32:
       return launch_c2r_filter_bs_t<kF32>(A_);

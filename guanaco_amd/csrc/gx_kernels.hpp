@@ -5,6 +5,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+#include "guanaco_hip.h"
 
 namespace gx {
 
@@ -27,6 +30,7 @@ struct BpArgs {
   int ndf, N, T, NYB, W, Z, Y;
   int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
   int y_base;                // vol holds rows [y_base, ...)
+  int slab_yb0;              // slab holds row blocks [slab_yb0, ...)
   float *vol;                // [y - y_base][z][x]
   // per-row rotation centres: row_off[t * Y_tab + y] is the tilt's offset for
   // row y (kF32: the reference's offset; reduced: offset - 1/2), or null
@@ -57,9 +61,12 @@ hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
                               int XB, int ntilts, float2 *F, uint32_t *tmax,
                               int t0, int M, const BluesteinTable &bx,
                               hipStream_t s);
+// ylo, yhi: the rows of G written (the band the row pass reads; yhi < 0:
+// every row).
 hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                           int Y, int XB, int ndf, int ntilts, int M,
-                          const BluesteinTable &by, hipStream_t s);
+                          const BluesteinTable &by, hipStream_t s,
+                          int ylo = 0, int yhi = -1);
 hipError_t launch_rows_c2r_bs(const float2 *G, int X, int Y, int XB, int ndf,
                               int ntilts, float *out, int M,
                               const BluesteinTable &bx, hipStream_t s);
@@ -69,7 +76,7 @@ hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
                                      const uint32_t *tmax, float nrm_mul,
                                      int store, void *slab, int NYB, int YB,
                                      int M, const BluesteinTable &bx,
-                                     hipStream_t s);
+                                     hipStream_t s, int y0 = 0, int ny = -1);
 
 // CTF parameters in the reference's units (ctf.h:167-178): A and rad
 struct CtfParams {
@@ -93,7 +100,8 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
                            int XB, int ntilts, float2 *F, uint32_t *tmax,
                            int t0, hipStream_t s);
 hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
-                       int XB, int ndf, int ntilts, hipStream_t s);
+                       int XB, int ndf, int ntilts, hipStream_t s,
+                       int ylo = 0, int yhi = -1);
 // Same column pass with a real transfer function (one "defocus").
 hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
                                 float2 *G, int Y, int XB, int ntilts,
@@ -103,7 +111,7 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
-                                  hipStream_t s);
+                                  hipStream_t s, int y0 = 0, int ny = -1);
 hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
                            int ntilts, float *out, hipStream_t s);
 hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
@@ -146,5 +154,23 @@ hipError_t synth_projections(const SynthArgs *a, float *d_out, hipStream_t s);
 
 // Twiddle table W[j] = exp(-2 pi i j / kTwN) for the current device.
 const float2 *twiddles(hipError_t *err);
+
+// ---- host tables shared by the C-ABI translation units (gx_capi.hip) ----
+
+// Per-tilt constants {c, s, offset, scale} of reconstructor.cpp:29-47.
+void tilt_geometry(size_t T, const float *angles, float centre, float *coef);
+// Slab selection factors of reconstructor.cpp:29-32.
+void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
+                          float *dscale, float *doffset);
+// The reference's ramp filter (filter.cpp:30-45) for N samples on the
+// L-point grid, with the 1/L of the unnormalised inverse: L/2 + 1 values.
+std::vector<float> ramp_table(size_t N, size_t L);
+// Filter transform length of N samples: a power of two >= 2N - 1, >= 8.
+size_t filter_length(size_t N);
+// Config checks of config.h:66-75 and the filter's size limit (GX_OK or a
+// status with the message set).
+int check_recon_cfg(const gx_recon_cfg *c);
+// Sets the thread's error message; returns code.
+int fail_status(int code, const char *msg);
 
 }  // namespace gx

@@ -281,6 +281,15 @@ void gather_rows(const VolumeRequest &r, const float *in, std::size_t y0,
 
 }  // namespace
 
+std::size_t volume_input_bytes(const VolumeRequest &r) {
+  validate(r);
+  const bool ctf = r.layout == Layout::projections && !r.defocus.empty();
+  // the device corrects (T, Y, X) projections into num_defocus slabs;
+  // corrected and sinogram input holds every slab
+  const std::size_t planes = ctf ? r.num_angles : r.num_angles * r.num_defocus;
+  return planes * r.num_rows * r.num_pixels * sizeof(float);
+}
+
 std::size_t volume_plan_bytes(const VolumeRequest &r) {
   validate(r);
   gx_volume_desc d = make_desc(r, r.num_rows);
@@ -300,32 +309,48 @@ void reconstruct_volume(const VolumeRequest &r, const float *input,
   std::size_t free_b = 0, total_b = 0;
   check_status(gx_mem_get_info(&free_b, &total_b));
   const std::size_t budget = free_b / 10 * 9;
-  const std::size_t in_row = T * K * X * sizeof(float);  // input per row
-  const std::size_t vol_row = Z * W * sizeof(float);     // volume per row
+  // input per detector row
+  const std::size_t in_row = volume_input_bytes(r) / Y;
+  const std::size_t vol_row = Z * W * sizeof(float);  // volume per row
 
   if (ctf) {
-    // The 2D correction needs whole projections: one plan for all rows, the
-    // volume written in bands that fit beside it.
+    // The 2D correction needs whole projections: they stay on the device,
+    // and one plan holds the filtered slabs of as many rows as fit (every
+    // row, else bands of about Y / n rows, multiples of 32); each band is
+    // backprojected into volume bands that fit beside it.
+    const std::size_t in_b = Y * in_row;
+    GUANACO_ASSERT(in_b < budget);
     gx_volume_desc d = make_desc(r, Y);
-    std::size_t plan_b = 0;
-    check_status(gx_volume_plan_bytes(&d, &plan_b));
-    GUANACO_ASSERT(plan_b + Y * in_row < budget);
+    std::size_t band = Y, plan_b = 0;
+    for (std::size_t n = 1;; ++n) {
+      band = std::min<std::size_t>(Y, ((Y + n - 1) / n + 31) / 32 * 32);
+      d.band_rows = band == Y ? 0 : band;
+      check_status(gx_volume_plan_bytes(&d, &plan_b));
+      if (plan_b + in_b + std::min<std::size_t>(band, 32) * vol_row <= budget)
+        break;
+      GUANACO_ASSERT(band > 32);
+    }
     Plan plan(d);
-    Dev d_in(Y * in_row);
-    check_status(gx_memcpy_htod(d_in.p, input, Y * in_row, nullptr));
-    check_status(gx_volume_filter(plan.p, (const float *)d_in.p, nullptr));
+    Dev d_in(in_b);
+    check_status(gx_memcpy_htod(d_in.p, input, in_b, nullptr));
     std::size_t yb = 0;
     check_status(gx_volume_rows_per_block(plan.p, &yb));
-    std::size_t band = (budget - plan_b - Y * in_row) / vol_row;
-    band = std::min<std::size_t>(Y, band / yb * yb);
-    GUANACO_ASSERT(band > 0);
-    Dev d_vol(band * vol_row);
-    for (std::size_t y0 = 0; y0 < Y; y0 += band) {
-      const std::size_t y1 = std::min(Y, y0 + band);
-      check_status(gx_volume_backproject_rows(plan.p, y0, y1,
-                                              (float *)d_vol.p, nullptr));
-      check_status(gx_memcpy_dtoh(volume + y0 * Z * W, d_vol.p,
-                                  (y1 - y0) * vol_row, nullptr));
+    // volume rows per copy-out: the whole band, else multiples of 32 rows
+    std::size_t vband = (budget - plan_b - in_b) / vol_row;
+    vband = vband >= band ? band : vband / 32 * 32;
+    GUANACO_ASSERT(vband > 0 && (vband % yb == 0 || vband == Y));
+    Dev d_vol(vband * vol_row);
+    for (std::size_t s0 = 0; s0 < Y; s0 += band) {
+      const std::size_t s1 = std::min(Y, s0 + band);
+      check_status(gx_volume_filter_rows(plan.p, (const float *)d_in.p, s0,
+                                         s1, nullptr));
+      for (std::size_t y0 = s0; y0 < s1; y0 += vband) {
+        const std::size_t y1 = std::min(s1, y0 + vband);
+        check_status(gx_volume_backproject_rows(plan.p, y0, y1,
+                                                (float *)d_vol.p, nullptr));
+        check_status(gx_memcpy_dtoh(volume + y0 * Z * W, d_vol.p,
+                                    (y1 - y0) * vol_row, nullptr));
+      }
     }
     return;
   }

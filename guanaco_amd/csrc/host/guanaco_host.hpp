@@ -139,6 +139,11 @@ struct VolumeRequest {
   int gpu_index = -1;
 };
 
+// Bytes of host input the request reads: T x Y x X floats of projections
+// corrected on the device, T x num_defocus x Y x X of corrected or sinogram
+// input.
+std::size_t volume_input_bytes(const VolumeRequest &r);
+
 // Device bytes a plan for this request holds (admission footprint).
 std::size_t volume_plan_bytes(const VolumeRequest &r);
 

@@ -115,7 +115,8 @@ int gx_filter(const float *rows, float *out, size_t nrows, size_t num_pixels,
  * guanaco::correct (correct.h:77-95): image (ysize x xsize), ctf complex64
  * interleaved (num_ctf x ysize x xsize), rec (num_ctf x ysize x xsize)
  * overwritten with Re(IFFT2_unnorm(FFT2(image) * -sign(Im ctf))).
- * xsize and ysize must be powers of two in [8, 4096] and [4, 4096]. */
+ * Any xsize and ysize in [1, 4096]: powers of two (xsize >= 8, ysize >= 4)
+ * take the power-of-two transforms, other sizes Bluestein transforms. */
 int gx_corr(const float *image, const float *ctf, float *rec, size_t xsize,
             size_t ysize, size_t num_ctf, int gpu_index);
 
@@ -182,6 +183,15 @@ typedef struct gx_volume_desc {
    * NULL for `centre` on every row. With per-row centres the plan
    * backprojects with the direct kernel, taps gathered per row. */
   const float *centres;
+  /* Detector rows whose filtered slabs the plan holds at a time (rounded up
+   * to a multiple of 32), or 0 for every row. A banded plan runs the path
+   * band by band (gx_volume_run): each band is filtered (for CTF-corrected
+   * input, the whole 2D correction again, its row pass only for the band)
+   * and backprojected into its rows of the volume, so FP32 slabs of volumes
+   * whose slabs do not fit beside them fit. No reference counterpart: the
+   * reference filters and backprojects one slice at a time (mp.py:106-154),
+   * and slice y needs row y of every slab only (reconstructor.cpp:50,72). */
+  size_t band_rows;
 } gx_volume_desc;
 
 typedef struct gx_volume_plan gx_volume_plan;
@@ -198,13 +208,23 @@ int gx_volume_plan_destroy(gx_volume_plan *plan);
 int gx_volume_run(gx_volume_plan *plan, const float *d_in, float *d_vol,
                   void *stream);
 /* Stages of gx_volume_run, for timing: filter/correct into the plan's slabs,
- * and backprojection of the slabs. */
+ * and backprojection of the slabs (plans that hold every row). */
 int gx_volume_filter(gx_volume_plan *plan, const float *d_in, void *stream);
 int gx_volume_backproject(gx_volume_plan *plan, float *d_vol, void *stream);
+/* Filter/correct detector rows [y0, y1) only into the plan's slabs (y0 a
+ * multiple of the plan's rows per block, y1 - y0 at most its band rows);
+ * d_in holds every row. Later backprojections take rows of this band. */
+int gx_volume_filter_rows(gx_volume_plan *plan, const float *d_in, size_t y0,
+                          size_t y1, void *stream);
+/* Rows held per band (the plan's rows when unbanded), and the band the
+ * slabs hold after the last filter pass. */
+int gx_volume_plan_band(gx_volume_plan *plan, size_t *band_rows, size_t *y0,
+                        size_t *y1);
 /* Backprojection of detector rows [y0, y1) only, into d_vol holding those
  * rows ((y1 - y0) x Z x W). y0 must be a multiple of the plan's rows per
- * block, y1 too unless it is num_rows: volumes larger than the free HBM are
- * written band by band. */
+ * block, y1 too unless it is num_rows, and the rows must lie in the band
+ * last filtered: volumes larger than the free HBM are written band by
+ * band. */
 int gx_volume_backproject_rows(gx_volume_plan *plan, size_t y0, size_t y1,
                                float *d_vol, void *stream);
 int gx_volume_rows_per_block(gx_volume_plan *plan, size_t *rows);

@@ -233,6 +233,45 @@ def corr(image, ctf, rec=None, precision="f32"):
     return out
 
 
+def flip_mask(ctf):
+    """The phase flip of correct.h:30-39 as a real factor per frequency:
+    phase_flip(x, H) = -x sign(Im H), the sign read from the complex64 CTF as
+    correct.cpp:57-89 reads it (sign(0) = 0). int8 in {-1, 0, 1}, the shape
+    of `ctf`."""
+    c = numpy.asarray(ctf, dtype=numpy.complex64)
+    return (-numpy.sign(c.imag)).astype(numpy.int8)
+
+
+def corr_rows(image, masks, rows):
+    """correct.cpp:30-90 in FP64 at a few detector rows only.
+
+    The correction of one image with flip factors m_k (flip_mask) is
+    Re(IFFT2_unnorm(FFT2(image) m_k)) (correct.cpp:57-89: forward FFT,
+    phase flip, unnormalised inverse, real part). Its rows y_r are
+    Re(IFFT_X_unnorm(E F m_k)) with F = FFT2(image) and
+    E[r, ky] = exp(2 pi i ky y_r / Y) the inverse column DFT evaluated at
+    those rows only, so the sampled rows cost one 2D FFT plus a
+    (rows x Y) (Y x X) product per defocus instead of a 2D inverse each.
+    Same arithmetic as corr(precision="f64") up to FP64 rounding (numpy's
+    pocketfft in place of the oracle's FFT).
+
+    masks: (ndf, Y, X) int8; returns (ndf, len(rows), X) float64."""
+    img = numpy.asarray(image, dtype=numpy.float64)
+    masks = numpy.asarray(masks)
+    ndf, Y, X = masks.shape
+    assert img.shape == (Y, X)
+    rows = numpy.asarray(rows, dtype=numpy.int64)
+    F = numpy.fft.fft2(img)
+    ky = numpy.arange(Y, dtype=numpy.int64)
+    # exact phases: (ky * y) mod Y in integers before the exponential
+    E = numpy.exp(2j * numpy.pi * ((rows[:, None] * ky[None, :]) % Y) / Y)
+    out = numpy.empty((ndf, rows.size, X), numpy.float64)
+    for k in range(ndf):
+        A = E @ (F * masks[k])
+        out[k] = (numpy.fft.ifft(A, axis=1) * X).real
+    return out
+
+
 # ---- ramp filter -----------------------------------------------------------
 
 
@@ -444,7 +483,10 @@ def reconstruct(tomogram, angles, centre=None, pixel_size=1, min_defocus=0,
     the square result there); `slices` restricts to a subset of y-slices.
     """
     assert tomogram.ndim in (3, 4)
-    sino = tomogram.astype("float32", copy=False)
+    # reconstruct.py:160 casts to float32; the FP64 twin keeps the input's
+    # precision (float64 input stays float64 through the weights and filter)
+    dt = numpy.float32 if precision == "f32" else numpy.float64
+    sino = tomogram.astype(dt, copy=False)
     if not sinogram_order:
         sino = numpy.swapaxes(sino, 0, -2)
     N = sino.shape[-1]
@@ -452,7 +494,6 @@ def reconstruct(tomogram, angles, centre=None, pixel_size=1, min_defocus=0,
     centre = get_centre(sino.shape, centre)
     weights = get_weights(angles, angular_weights)
     ys = range(sino.shape[0]) if slices is None else slices
-    dt = numpy.float32 if precision == "f32" else numpy.float64
     out = numpy.zeros((len(ys), H, N), dtype=dt)
     for n, i in enumerate(ys):
         s = sino[i]
@@ -460,7 +501,7 @@ def reconstruct(tomogram, angles, centre=None, pixel_size=1, min_defocus=0,
             s = -s
         # The reference's ascontiguousarray can alias a contiguous caller
         # array and then scale it in place; the oracle always copies.
-        s = numpy.array(s, dtype=numpy.float32, order="C", copy=True)
+        s = numpy.array(s, dtype=dt, order="C", copy=True)
         if weights is not None:
             if s.ndim == 3:
                 for j in range(s.shape[1]):
@@ -468,8 +509,6 @@ def reconstruct(tomogram, angles, centre=None, pixel_size=1, min_defocus=0,
             else:
                 for j in range(s.shape[0]):
                     s[j, :] *= weights[j]
-        if precision != "f32":
-            s = s.astype(numpy.float64)
         out[n] = recon(s, (H, N), angles, centre[i], pixel_size, min_defocus,
                        max_defocus, precision=precision,
                        f32_decisions=f32_decisions)

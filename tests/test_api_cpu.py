@@ -147,3 +147,27 @@ def test_corr_shape_assertions():
         G.corr(numpy.zeros((8, 8), numpy.float32),
                numpy.zeros((8, 4), numpy.complex64),
                numpy.zeros((8, 8), numpy.float32), "gpu")
+
+
+def test_volume_input_bytes_of_device_corrected_projections():
+    # The fused path reads the (T, Y, X) projections once; num_defocus only
+    # multiplies corrected (T, ndf, Y, X) and sinogram (Y, ndf, T, X) input.
+    ext = G.detail
+    T, Y, X, K = 7, 16, 64, 8
+    assert ext.volume_input_bytes(X, Y, T, K, 0, True) == 4 * T * Y * X
+    assert ext.volume_input_bytes(X, Y, T, K, 1, False) == 4 * T * K * Y * X
+    assert ext.volume_input_bytes(X, Y, T, K, 2, False) == 4 * T * K * Y * X
+    assert ext.volume_input_bytes(X, Y, T, 1, 0, False) == 4 * T * Y * X
+
+
+def test_reconstruct_volume_checks_array_sizes():
+    # a wrong-sized input or volume is refused before any device work
+    T, Y, X, K = 5, 8, 32, 4
+    angles = numpy.zeros(T, numpy.float32)
+    vol = numpy.zeros((Y, 8, X), numpy.float32)
+    # (T, K, Y, X) handed to the path that corrects (T, Y, X) projections
+    wide = numpy.zeros((T, K, Y, X), numpy.float32)
+    with pytest.raises(RuntimeError, match="volume_input_bytes"):
+        G.detail.reconstruct_volume(wide, vol, angles, 0, X / 2.0,
+                                    defocus=numpy.full(K, 1e4),
+                                    wavelength=0.02, Cs=2.7e7)

@@ -372,3 +372,74 @@ def test_get_centre():
     assert c.shape == (10,) and numpy.all(c == 32.0) and c.dtype == numpy.float32
     c = O.get_centre((5, 10, 64), 20.5, sinogram_order=False)
     assert c.shape == (10,) and numpy.all(c == 20.5)
+
+
+@pytest.mark.parametrize("Y,X,rows", [(16, 12, [0, 5, 15]),
+                                      (64, 48, [0, 31, 63]),
+                                      (30, 40, [1, 2, 29])])
+def test_corr_rows_equals_full_fp64_correction(Y, X, rows):
+    # the sampled-row FP64 correction is the full one at those rows
+    rng = numpy.random.default_rng(Y * X)
+    img = rng.standard_normal((Y, X)).astype(numpy.float32) + 1.0
+    l = O.electron_wavelength(300000)
+    ctf = numpy.stack([O.ctf_simple_grid(l, df, 2.7e7, 300.0, 0.3, 0.1, X, Y,
+                                         1.5) for df in (15000.0, 20000.0)]
+                      ).astype(numpy.complex64)
+    full = O.corr(img, ctf, precision="f64")
+    got = O.corr_rows(img, O.flip_mask(ctf), rows)
+    assert numpy.allclose(got, full[:, rows], rtol=0, atol=1e-9 * abs(
+        full).max())
+
+
+def test_flip_mask_special_cases():
+    # df = Cs = 0, phi = 90 deg: H = exp(i pi / 2) = i, flip -1 everywhere;
+    # df = Cs = phi = 0: H = 1, Im H = 0, flip 0 (the reference's sign(0))
+    l = O.electron_wavelength(300000)
+    h = O.ctf_simple_grid(l, 0.0, 0.0, 0, 0, math.pi / 2, 8, 4, 1.0)
+    assert (O.flip_mask(h) == -1).all()
+    h0 = O.ctf_simple_grid(l, 0.0, 0.0, 0, 0, 0.0, 8, 4, 1.0)
+    assert (O.flip_mask(h0) == 0).all()
+
+
+def test_bp_all_fp64_decisions_closed_form_and_slabs():
+    # With FP64 decisions (slab choice and taps from FP64 coordinates) the
+    # linear sinogram is interpolated exactly: the closed form to FP64
+    # rounding on every voxel whose taps are inside; and the hand-computed
+    # slabs of test_bp_slab_index_hand_computed hold (no tie is involved).
+    N, T = 64, 7
+    a, b = 0.25, 0.5
+    angles = numpy.linspace(-1.0, 1.0, T).astype(numpy.float32)
+    row = a + b * numpy.arange(N, dtype=numpy.float64)
+    rec = O.project(numpy.tile(row, (T, 1)), (N, N), angles, N / 2,
+                    precision="f64", f32_decisions=False)
+    j, i = numpy.mgrid[0:N, 0:N]
+    x = i - N / 2 + 0.5
+    z = j - N / 2 + 0.5
+    expect = numpy.zeros((N, N))
+    for th in angles.astype(numpy.float64):
+        p = x * math.cos(th) - z * math.sin(th) + N / 2
+        expect += math.pi / (2 * T) * (a + b * (p - 0.5))
+    inner = (x ** 2 + z ** 2) < (N / 2 - 4) ** 2
+    assert numpy.allclose(rec[inner], expect[inner], rtol=1e-12)
+    sino = numpy.stack([numpy.full((1, 8), k + 1.0) for k in range(3)])
+    rec = O.project(sino, (4, 8), [0.0], 4.0, 1.0, -2.0, 2.0,
+                    precision="f64", f32_decisions=False)
+    for jj, k in enumerate([2, 2, 1, 0]):
+        assert numpy.allclose(rec[jj, :-1], math.pi / 2 * (k + 1))
+        assert rec[jj, -1] == 0
+
+
+def test_fp64_reconstruct_keeps_fp64_input():
+    # a float64 tomogram is not rounded to float32 on the FP64 path
+    N, T = 32, 5
+    angles = numpy.linspace(-1.0, 1.0, T).astype(numpy.float32)
+    rng = numpy.random.default_rng(8)
+    tomo = rng.standard_normal((T, 2, N)) * (1 + 1e-9)
+    a = O.reconstruct(tomo, angles, precision="f64")
+    b = O.reconstruct(tomo.astype(numpy.float32).astype(numpy.float64),
+                      angles, precision="f64")
+    direct = O.recon(numpy.ascontiguousarray(tomo[:, 1, :]), (N, N), angles,
+                     N / 2, precision="f64")
+    assert numpy.array_equal(a[1], direct)
+    assert not numpy.array_equal(a, b)
+
