@@ -45,7 +45,6 @@ os.environ["MKL_NUM_THREADS"] = "1"
 import argparse  # noqa: E402
 import json  # noqa: E402
 import math  # noqa: E402
-import multiprocessing as mp  # noqa: E402
 import platform  # noqa: E402
 import sys  # noqa: E402
 import threading  # noqa: E402
@@ -751,7 +750,7 @@ def run_workload(C, name, w, args, pool, stream, headline):
     if not args.no_check or (headline and not args.no_cpu):
         proj = d_in.to_host()
     host = None
-    if name in ("c2",) and not args.no_host:
+    if name == "c2" and not args.no_host and proj is not None:
         host = host_path(w, angles, proj, "f16")
         log("host buffers (f16) %.1f ms/request incl. PCIe" % (
             host["ms_per_request"]))
@@ -831,7 +830,8 @@ def main():
     nworkers = min(WORKERS_MAX, host_info()["usable_cpus"])
     pool = None
     if not args.no_check or not args.no_cpu:
-        pool = mp.get_context("fork").Pool(nworkers)
+        from oracle import sampled
+        pool = sampled.make_pool(nworkers, "fork")
 
     from guanaco_amd import _capi as C
     if C.device_count() < 1:

@@ -901,9 +901,17 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
     const size_t per_tilt = nspec * sizeof(float2) * (1 + ndf);
     chunk = d->tilt_chunk;
     if (chunk == 0) {
-      const size_t budget = (size_t)4 << 30;
-      chunk = budget / per_tilt;
-      if (chunk < 1) chunk = 1;
+      // the tilts whose spectra fit in 4 GB; for long columns (two
+      // workgroups per CU, 512 at once) raised, within 16 GB, until the
+      // column pass's grid is 16 such rounds, so that its last, partly
+      // idle round costs little
+      const size_t budget = (size_t)4 << 30, cap = (size_t)16 << 30;
+      chunk = std::max<size_t>(1, budget / per_tilt);
+      if (Y >= 2048 && cols_pow2(Y)) {
+        const size_t wg = (Y == 4096 ? 2 : 1) * (size_t)ps->XB;
+        const size_t want = (16 * 512 + wg - 1) / wg;
+        chunk = std::max(chunk, std::min(want, cap / per_tilt));
+      }
     }
     if (chunk > T) chunk = T;
     ps->f_bytes = nspec * sizeof(float2) * chunk;
@@ -1161,8 +1169,11 @@ extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
   if (p->ctf) {
     // The 2D correction needs whole projections: every row is transformed,
     // the column pass writes and the row pass reads only the band's rows.
-    for (int t0 = 0; t0 < p->T; t0 += p->chunk) {
-      const int tc = p->T - t0 < p->chunk ? p->T - t0 : p->chunk;
+    // balanced launches of at most `chunk` tilts
+    const int nch = (p->T + p->chunk - 1) / p->chunk;
+    for (int c = 0; c < nch; ++c) {
+      const int t0 = (int)((long)p->T * c / nch);
+      const int tc = (int)((long)p->T * (c + 1) / nch) - t0;
       const float *img = d_in + (size_t)t0 * Y * X;
       if (p->rows_bs) {
         GX_HIP(launch_rows_r2c_bs(img, Y * X, p->X, p->Y, p->XB, tc,

@@ -18,6 +18,23 @@ __all__ = ["correct_projections", "correct_file", "get_defocus_steps",
            "get_corrected_array", "get_ctf", "get_ctf_array"]
 
 
+# Detector sizes of the gfx950 kernels: the correction's 2D transforms and
+# the ramp filter's 2N-point transforms take up to 4096 rows and columns.
+MAX_DETECTOR = 4096
+
+
+def check_detector_size(width, height=None):
+    """Raises RuntimeError for detectors the gfx950 path does not take:
+    width (the filtered rows, and the corrected rows) and, with the 2D
+    correction, height above MAX_DETECTOR."""
+    too = [(n, v) for n, v in (("width", width), ("height", height))
+           if v is not None and v > MAX_DETECTOR]
+    if too:
+        raise RuntimeError(
+            "detector %s exceeds the %d pixels the gfx950 kernels take" % (
+                " and ".join("%s %d" % t for t in too), MAX_DETECTOR))
+
+
 def get_centre(shape, centre, sinogram_order=True):
     """Per-row rotation centres (correct.py:33-41)."""
     if not sinogram_order:
@@ -164,6 +181,7 @@ def correct_file(input_filename, output_filename, centre=None, energy=None,
         assert voxel_size["x"] == voxel_size["y"]
         pixel_size = float(voxel_size["x"])
         projections = infile.data
+        check_detector_size(projections.shape[2], projections.shape[1])
         centre = get_centre(projections.shape, centre, False)
         if num_defocus is None or num_defocus == 0:
             num_defocus = 1

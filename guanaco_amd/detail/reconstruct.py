@@ -14,7 +14,9 @@ import numpy
 import guanaco_amd.detail as ext
 from guanaco_amd import mrc
 from guanaco_amd.detail import mp
-from guanaco_amd.detail.correct import correct_projections, get_defocus_steps
+from guanaco_amd.detail.correct import (check_detector_size,
+                                        correct_projections,
+                                        get_defocus_steps)
 
 __all__ = ["reconstruct", "reconstruct_projections", "reconstruct_file",
            "reconstruct_tomopy", "get_centre", "compute_angular_weights",
@@ -132,6 +134,7 @@ def reconstruct_projections(projections, angles, centre=None, pixel_size=1,
     projections = numpy.ascontiguousarray(projections, dtype=numpy.float32)
     assert projections.ndim == 3, "projections must be (T, Y, X)"
     T, Y, X = projections.shape
+    check_detector_size(X, Y if defocus is not None else None)
     c = get_centre(projections.shape, centre, False)
     weights = get_weights(angles, angular_weights).astype(numpy.float32)
     if transform == "minus":
@@ -264,6 +267,10 @@ def reconstruct_file(input_filename, output_filename,
             # the third-party algorithms: checked before any output is made
             _tomopy()
         run_fused = fused and not corrected and defocus is not None
+        # sizes are checked before any output file is made
+        check_detector_size(X, projections.shape[-2]
+                            if (not corrected and defocus is not None)
+                            else None)
         if not corrected and defocus is not None and not run_fused:
             projections, min_defocus, max_defocus = correct_projections(
                 projections, centre, pixel_size, energy, defocus,

@@ -25,10 +25,28 @@ import numpy
 from oracle import oracle as O
 
 
-def defocus_list(df, ndf, width, height, ps):
+def worker_init():
+    """Pool initializer: one BLAS/OpenMP thread per worker (each worker is
+    one core; the environment may ask libraries for many threads)."""
+    try:
+        from threadpoolctl import threadpool_limits
+        threadpool_limits(1)
+    except ImportError:
+        pass
+
+
+def make_pool(n, context="spawn"):
+    """A pool of n single-threaded oracle workers."""
+    import multiprocessing as mp
+    return mp.get_context(context).Pool(n, initializer=worker_init)
+
+
+def defocus_list(df, ndf, width, height, ps, centre=None):
     """Absolute defoci of the slabs and the relative range
-    (get_defocus_steps, correct.py:44-93, centre width / 2 on every row)."""
-    centre = numpy.full(height, width / 2.0, numpy.float32)
+    (get_defocus_steps, correct.py:44-93; centre: one rotation centre for
+    every row, width / 2 by default)."""
+    centre = numpy.full(height, width / 2.0 if centre is None else centre,
+                        numpy.float32)
     mn, mx, st, n = O.get_defocus_steps(df, ndf, None, width, centre, ps)
     return [mn + d * st for d in range(n)], mn - df, mx - df
 
@@ -75,9 +93,10 @@ def recon_task(a):
 
 
 def reference_rows(pool, proj, angles, rows, depth, ps=1.0, df=None, ndf=1,
-                   phi=0.0, Cs=2.7e7, fp32=True, tilts=None):
+                   phi=0.0, Cs=2.7e7, fp32=True, tilts=None, centre=None):
     """Oracle slices at `rows` of the reconstruction of (T, Y, X)
-    projections, corrected with ndf defoci around df (None: no correction).
+    projections, corrected with ndf defoci around df (None: no correction),
+    rotation centre `centre` on every row (X / 2 by default).
 
     Returns {"f64": (nrows, depth, X), "f64_all_f64_decisions": ...,
     "f32": ... (when fp32)} and, with `tilts`, "corrected64" (the FP64
@@ -94,7 +113,7 @@ def reference_rows(pool, proj, angles, rows, depth, ps=1.0, df=None, ndf=1,
             s32 = numpy.ascontiguousarray(proj[:, None, rows, :],
                                           dtype=numpy.float32)
         else:
-            dfs, mn, mx = defocus_list(df, ndf, X, Y, ps)
+            dfs, mn, mx = defocus_list(df, ndf, X, Y, ps, centre)
             masks = pool.map(mask_task, [(X, Y, ps, phi, Cs, d) for d in dfs])
             fd, tmp = tempfile.mkstemp(suffix=".npy")
             os.close(fd)
@@ -115,7 +134,7 @@ def reference_rows(pool, proj, angles, rows, depth, ps=1.0, df=None, ndf=1,
             a = numpy.ascontiguousarray(numpy.swapaxes(s[:, :, i, :], 0, 1))
             return a[0] if a.shape[0] == 1 else a
 
-        centre = X / 2.0
+        centre = X / 2.0 if centre is None else float(centre)
         tasks, keys = [], []
         for i in range(rows.size):
             for key, s, prec, dec in (("f64", s64, "f64", True),

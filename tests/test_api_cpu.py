@@ -171,3 +171,18 @@ def test_reconstruct_volume_checks_array_sizes():
         G.detail.reconstruct_volume(wide, vol, angles, 0, X / 2.0,
                                     defocus=numpy.full(K, 1e4),
                                     wavelength=0.02, Cs=2.7e7)
+
+
+def test_detectors_above_4096_are_refused_before_any_work(tmp_path):
+    # the gfx950 kernels take up to 4096 rows and columns; larger detectors
+    # raise a RuntimeError naming the limit (no GPU needed to refuse them)
+    from guanaco_amd.detail.correct import check_detector_size
+    check_detector_size(4096, 4096)
+    with pytest.raises(RuntimeError, match="width 5760 exceeds the 4096"):
+        check_detector_size(5760, 4092)
+    with pytest.raises(RuntimeError, match="height 5000"):
+        check_detector_size(1000, 5000)
+    proj = numpy.zeros((3, 8, 4100), numpy.float32)
+    with pytest.raises(RuntimeError, match="4096"):
+        G.reconstruct_projections(proj, numpy.zeros(3, numpy.float32),
+                                  defocus=20000.0, num_defocus=2)
