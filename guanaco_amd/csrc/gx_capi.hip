@@ -254,12 +254,27 @@ gx::BluesteinTable bluestein_view(const float2 *d, size_t n, size_t M) {
 
 }  // namespace
 
-// The twiddle table of every transform, one copy per GPU in a device
-// global: filled on the first use on a device, never allocated or freed.
-static __device__ __align__(256) float2 twiddle_table[gx::tw_table_len()];
-
 namespace gx {
 
+std::vector<float2> twiddle_host_table() {
+  // W^j, then the per-thread blocks of every transform shape (gx_fft.hpp)
+  std::vector<float2> h(tw_table_len());
+  for (int j = 0; j < kTwN; ++j) {
+    const double a = 2.0 * M_PI * (double)j / (double)kTwN;
+    h[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
+  }
+  for (int ln = 1; ln <= 14; ++ln)
+    for (int e = 2; e <= 16; e *= 2)
+      if (tw_count(1 << ln, e) > 0)
+        tw_fill_block(1 << ln, e, h.data(),
+                      h.data() + kTwN + tw_block_offset(1 << ln, e));
+  return h;
+}
+
+// One table per GPU in ordinary (cached) device memory, allocated on the
+// first use on that device and held for the life of the process. (A device
+// global array measured 12-34 % slower filter passes: the FFT kernels read
+// their twiddles on every stage.)
 const float2 *twiddles(hipError_t *err) {
   static std::mutex mu;
   static float2 *tables[64] = {};
@@ -272,26 +287,17 @@ const float2 *twiddles(hipError_t *err) {
   }
   std::lock_guard<std::mutex> lock(mu);
   if (tables[dev] == nullptr) {
-    // W^j, then the per-thread blocks of every transform shape (gx_fft.hpp)
-    std::vector<float2> h(tw_table_len());
-    for (int j = 0; j < kTwN; ++j) {
-      const double a = 2.0 * M_PI * (double)j / (double)kTwN;
-      h[j] = make_float2((float)std::cos(a), (float)-std::sin(a));
-    }
-    for (int ln = 1; ln <= 14; ++ln)
-      for (int e = 2; e <= 16; e *= 2)
-        if (tw_count(1 << ln, e) > 0)
-          tw_fill_block(1 << ln, e, h.data(),
-                        h.data() + kTwN + tw_block_offset(1 << ln, e));
-    // the device address of this GPU's copy of the array (its name is only
-    // a host-side stand-in)
-    void *d = nullptr;
-    *err = hipGetSymbolAddress(&d, HIP_SYMBOL(twiddle_table));
+    const std::vector<float2> h = twiddle_host_table();
+    float2 *d = nullptr;
+    *err = hipMalloc(&d, sizeof(float2) * h.size());
     if (*err != hipSuccess) return nullptr;
     *err = hipMemcpy(d, h.data(), sizeof(float2) * h.size(),
                      hipMemcpyHostToDevice);
-    if (*err != hipSuccess) return nullptr;
-    tables[dev] = (float2 *)d;
+    if (*err != hipSuccess) {
+      (void)hipFree(d);
+      return nullptr;
+    }
+    tables[dev] = d;
   }
   *err = hipSuccess;
   return tables[dev];

@@ -293,17 +293,26 @@ __device__ __forceinline__ float mask_value(M m) {
   }
 }
 
-// CW columns per workgroup (CB, or CB / 2 for 4096-point columns, whose
-// 4-column workgroups fill a CU alone: two then overlap each other's
-// barriers; C4 filter 340 -> 327 ms, while at 2048 points, where two
-// 4-column workgroups already share a CU, halving them was slower). With
-// CW < CB the grid
-// is 1D over (column half, column block, tilt), remapped so that each XCD
-// takes consecutive ones: the two halves of a column block, which share its
-// 32-byte row segments, meet in one L2.
-template <int Y, typename M, int CW>
+// CW columns per workgroup (WPE waves per SIMD bound the registers). With
+// CW < CB the grid is 1D over (column part, column block, tilt), remapped so
+// that each XCD takes consecutive ones: the parts of a column block, which
+// share its 32-byte row segments, meet in one L2. The launches use CW = CB:
+// a wave then stores 16 rows x 4 columns, whole 128-byte lines of G; with
+// one column per workgroup at 4096 points the partial-line stores doubled
+// the bytes written to HBM (4.5 GB per tilt at 32 defoci, 2.15 GB
+// compulsory).
+//
+// Registers: the forward spectrum (2E) stays live across the defocus loop;
+// the per-row store addresses and the twiddles (about 50 registers) would
+// too if the compiler hoisted them, and at the 128 registers of four waves
+// per SIMD they spilled (22-68 registers, up to 276 bytes per lane of
+// scratch, which made most of the pass's memory traffic). The thread
+// indices therefore pass through an empty asm each defocus, and the
+// twiddles are reloaded (from the cache) per inverse transform: 106-108
+// registers, no scratch.
+template <int Y, typename M, int CW, int WPE = 4>
 __global__ __launch_bounds__(CW *(Y / default_e(Y)))
-__attribute__((amdgpu_waves_per_eu(4))) void k_cols(
+__attribute__((amdgpu_waves_per_eu(WPE))) void k_cols(
     const float2 *__restrict__ F, const M *__restrict__ mask,
     float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw,
     int ylo, int yhi) {
@@ -343,6 +352,11 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
   constexpr bool kRun = sizeof(M) == 1 && sizeof(MaskRun) == E;
   const size_t run0 = ((size_t)cb * NT + tid) * CB + ci;  // in E-byte runs
   for (int k = 0; k < ndf; ++k) {
+    // the thread's row and column indices pass through an empty asm each
+    // defocus, so the per-row store addresses are formed in the iteration
+    // instead of being held across the loop
+    int tk = tid, ck = ci;
+    asm volatile("" : "+v"(tk), "+v"(ck));
     float2 w[E];
     if constexpr (kRun) {
       int8_t mv[E];
@@ -362,14 +376,19 @@ __attribute__((amdgpu_waves_per_eu(4))) void k_cols(
         w[m] = make_float2(v[m].x * f, v[m].y * f);
       }
     }
-    Fft<Y, E, 1, CW>::run(w, s + cl, tid, twy);
+    // the twiddles are loaded again for each defocus (from the cache) rather
+    // than held across the loop
+    typename Fft<Y, E, 1, CW>::Twiddles twk;
+    twk.load(tk, tw);
+    Fft<Y, E, 1, CW>::run(w, s + cl, tk, twk);
     // only the rows [ylo, yhi) that the row pass reads (a band of a
     // banded plan; every row otherwise)
-    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
+    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB +
+                  ((size_t)tk * CB + ck);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int y = tid + m * NT;
-      if (y >= ylo && y < yhi) dst[(size_t)y * CB + ci] = w[m];
+      const int y = tk + m * NT;
+      if (y >= ylo && y < yhi) dst[(size_t)(m * NT) * CB] = w[m];
     }
   }
 }
@@ -1146,20 +1165,26 @@ __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
   }
   bluestein<M, E, -1, CW>(v, s + c, tid, twm, by);
   for (int k = 0; k < ndf; ++k) {
+    // as in k_cols: indices laundered and twiddles reloaded per defocus, so
+    // that addresses and twiddles are not held across the loop
+    int tk = tid, ck = ci;
+    asm volatile("" : "+v"(tk), "+v"(ck));
     const int8_t *mk = mask + ((size_t)k * XB + cb) * Y * CB;
     float2 w[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int y = tid + m * NT;
-      const float f = y < Y ? mask_value(mk[(size_t)y * CB + ci]) : 0.0f;
+      const int y = tk + m * NT;
+      const float f = y < Y ? mask_value(mk[(size_t)y * CB + ck]) : 0.0f;
       w[m] = make_float2(v[m].x * f, v[m].y * f);
     }
-    bluestein<M, E, 1, CW>(w, s + c, tid, twm, by);
-    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB;
+    typename Fft<M, E, -1, CW>::Twiddles twk;
+    twk.load(tk, tw);
+    bluestein<M, E, 1, CW>(w, s + c, tk, twk, by);
+    float2 *dst = G + (((size_t)t * ndf + k) * XB + cb) * Y * CB + ck;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const int y = tid + m * NT;
-      if (y >= ylo && y < yhi) dst[(size_t)y * CB + ci] = w[m];
+      const int y = tk + m * NT;
+      if (y >= ylo && y < yhi) dst[(size_t)y * CB] = w[m];
     }
   }
 }
@@ -1290,7 +1315,6 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
   dim3 grid(XB, ntilts);
-  const dim3 grid2((unsigned)(2 * XB * ntilts));
   switch (Y) {
 #define C_(n)                                                              \
   case n:                                                                  \
@@ -1299,15 +1323,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
                        ndf, tw, ylo, yhi);                                 \
     break;
     C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
-    C_(2048)
-#undef C_
-#define C_(n)                                                              \
-  case n:                                                                  \
-    hipLaunchKernelGGL((k_cols<n, M, CB / 2>), grid2,                      \
-                       dim3(CB / 2 * (n / default_e(n))), 0, s, F, mask, G, \
-                       XB, ndf, tw, ylo, yhi);                             \
-    break;
-    C_(4096)
+    C_(2048) C_(4096)
 #undef C_
     default:
       return hipErrorInvalidValue;
@@ -1627,10 +1643,13 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int L, int Y, int T, int ndf,
                               const float *ramp, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, int store,
-                              void *slab, int NYB, int YB, hipStream_t s) {
-  hipError_t err;
-  const float2 *tw = twiddles(&err);
-  if (err != hipSuccess) return err;
+                              void *slab, int NYB, int YB, hipStream_t s,
+                              const float2 *tw) {
+  if (tw == nullptr) {
+    hipError_t err;
+    tw = twiddles(&err);
+    if (err != hipSuccess) return err;
+  }
   // 512- to 8192-point filters: four half-length transforms per row pair
   if (L >= 512 && L <= 8192 && YB >= 8) {
 #define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
@@ -1723,8 +1742,9 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  // CB columns of a block per workgroup up to 4096-point transforms, half a
-  // block at 8192 points (LDS)
+  // CB columns of a block per workgroup up to 2048-point transforms, one
+  // column at 4096 and 8192 points (1024-thread workgroups would cap the
+  // registers at 128 and spill)
   switch (M) {
 #define C_(n)                                                                 \
   case n:                                                                     \
@@ -1733,13 +1753,15 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                        XB, ndf, tw, by, ylo, yhi);                            \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
-    C_(4096)
 #undef C_
-    case 8192:
-      hipLaunchKernelGGL((k_cols_bs<8192, CB / 2>), dim3(XB * 2, ntilts),
-                         dim3((CB / 2) * (8192 / default_e(8192))), 0, s, F,
-                         mask, G, Y, XB, ndf, tw, by, ylo, yhi);
-      break;
+#define C_(n)                                                                 \
+  case n:                                                                     \
+    hipLaunchKernelGGL((k_cols_bs<n, 1>), dim3(XB * CB, ntilts),              \
+                       dim3(n / default_e(n)), 0, s, F, mask, G, Y, XB, ndf,  \
+                       tw, by, ylo, yhi);                                     \
+    break;
+    C_(4096) C_(8192)
+#undef C_
     default:
       return hipErrorInvalidValue;
   }

@@ -114,11 +114,14 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   hipStream_t s, int y0 = 0, int ny = -1);
 hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
                            int ntilts, float *out, hipStream_t s);
+// tw: a twiddle table (twiddle_host_table() values) to use instead of the
+// device's own (twiddles()).
 hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int L, int Y, int T, int ndf,
                               const float *ramp, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, int store,
-                              void *slab, int NYB, int YB, hipStream_t s);
+                              void *slab, int NYB, int YB, hipStream_t s,
+                              const float2 *tw = nullptr);
 hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int Y, int T, int ndf,
                               uint32_t *tmax, hipStream_t s);
@@ -152,8 +155,10 @@ struct SynthArgs {
 };
 hipError_t synth_projections(const SynthArgs *a, float *d_out, hipStream_t s);
 
-// Twiddle table W[j] = exp(-2 pi i j / kTwN) for the current device.
+// Twiddle table W[j] = exp(-2 pi i j / kTwN) for the current device
+// (allocated on its first use there), and its host values.
 const float2 *twiddles(hipError_t *err);
+std::vector<float2> twiddle_host_table();
 
 // ---- host tables shared by the C-ABI translation units (gx_capi.hip) ----
 
