@@ -17,8 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["gx_bp.hip", "gx_capi.hip", "gx_correct.hip", "gx_serve.hip",
-               "gx_synth.hip"]
+HIP_SOURCES = ["gx_bp.hip", "gx_capi.hip", "gx_correct.hip", "gx_noalloc.hip",
+               "gx_serve.hip", "gx_synth.hip"]
 HOST_SOURCES = ["host/guanaco_host.cpp", "ext/guanaco_ext.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -167,6 +167,7 @@ SIGNATURES = {
     "gx_graph_destroy": [_vp],
     "gx_tilt_geometry": [ctypes.POINTER(ReconCfg), _f32p],
     "gx_recon": [ctypes.POINTER(ReconCfg), _f32p, _f32p],
+    "gx_recon_noalloc": [ctypes.POINTER(ReconCfg), _vp, _vp, _vp],
     "gx_project": [ctypes.POINTER(ReconCfg), _f32p, _f32p],
     "gx_filter": [_f32p, _f32p, _sz, _sz, _int],
     "gx_corr": [_f32p, _f32p, _f32p, _sz, _sz, _sz, _int],

@@ -102,6 +102,20 @@ int gx_tilt_geometry(const gx_recon_cfg *cfg, float *coef);
  * (grid_height x grid_width, host) with the backprojection. */
 int gx_recon(const gx_recon_cfg *cfg, const float *sino, float *rec);
 
+/* No counterpart in the reference. gx_recon for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): d_sino (num_defocus x num_angles
+ * x num_pixels) and d_rec (grid_height x grid_width) are device buffers,
+ * d_rec is overwritten with gx_recon's result (the same kernels, bit for
+ * bit), d_sino is not modified. The filtered copy and the per-call tables
+ * live in a 64 MiB device-global scratch array per GPU (no hipMalloc or
+ * hipFree): sizes with num_defocus * num_angles * num_pixels * 4 bytes plus
+ * tables above 64 MiB are refused with GX_EINVAL. Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU race.
+ * Synchronizes `stream` (NULL: the default stream) on entry and return. */
+int gx_recon_noalloc(const gx_recon_cfg *cfg, const float *d_sino,
+                     float *d_rec, void *stream);
+
 /* Backprojection only (reconstructor.cpp:14-90) of an already filtered
  * sinogram; rec is overwritten (the reference zeroes it first). */
 int gx_project(const gx_recon_cfg *cfg, const float *filtered, float *rec);
