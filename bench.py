@@ -463,22 +463,22 @@ def cpu_baseline(w, proj, angles, pool, nworkers):
     total_n = total1 * (wall / one_round) / n
     info = host_info()
     return {
-        "value": X * Y * Z * T / total1,
+        "value": X * Y * Z * T / total_n,
         "unit": UNIT,
-        "cores": 1,
+        "cores": n,
         "kind": "port",
-        "sample": sample,
-        "seconds_estimated_whole_job": total1,
+        "sample": "%s; the same units run on %d worker processes at once "
+                  "(wall clock %.2f s for %.2f s of one-core work each) and "
+                  "the whole job extrapolated from that rate; workers = the "
+                  "CPUs this process may use, at most the GPU's 16-CPU share "
+                  "of the host" % (sample, n, wall, one_round),
+        "seconds_estimated_whole_job": total_n,
         **info,
-        "all_cores": {
-            "value": X * Y * Z * T / total_n,
+        "one_core": {
+            "value": X * Y * Z * T / total1,
             "unit": UNIT,
-            "cores": n,
-            "sample": "the same units on %d worker processes at once "
-                      "(wall clock %.2f s for %.2f s of one-core work each), "
-                      "extrapolated; worker count capped at the GPU's "
-                      "16-CPU share of the host" % (n, wall, one_round),
-            "seconds_estimated_whole_job": total_n,
+            "cores": 1,
+            "seconds_estimated_whole_job": total1,
         },
     }
 
@@ -898,8 +898,8 @@ def main():
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(w, proj, angles, pool, nworkers)
         log("CPU oracle %.3g updates/s on 1 core, %.3g on %d (%.1f s)" % (
-            line["cpu_baseline"]["value"],
-            line["cpu_baseline"]["all_cores"]["value"], nworkers,
+            line["cpu_baseline"]["one_core"]["value"],
+            line["cpu_baseline"]["value"], nworkers,
             time.perf_counter() - t0))
     if pool is not None:
         pool.close()
