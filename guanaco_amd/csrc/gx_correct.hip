@@ -417,6 +417,42 @@ __device__ __forceinline__ float2 load_pair_spectrum(const float2 *g, int X,
                      : make_float2(A.x + B.y, B.x - A.y);
 }
 
+// load_pair_spectrum for the E columns kx = tid + m NT of one thread, both
+// rows (ya, ya + 1) inside the image. In the [cb][y][ci] layout (CB = 4)
+// column kk of row y is element (kk / 4 * Y + y) * 4 + kk % 4, and row ya + 1
+// lies 32 bytes after row ya. Columns kx <= X / 2 (m < E / 2, and m = E / 2
+// at tid = 0) are read as they are, the others as the mirrored column
+// X - kx, conjugated; the element offsets are then
+//   kk = kx:      lo + m NT Y,         lo = ((tid / 4) Y + ya) 4 + tid % 4
+//   kk = X - kx:  hi + (X - m NT) Y,   hi = (ya - ceil(tid / 4) Y) 4 + (-tid) % 4
+// (the two agree at m = E / 2, tid = 0), i.e. one per-thread value plus a
+// uniform one. Byte offsets are 32-bit: one (tilt, defocus) spectrum holds
+// fewer than 2^28 elements.
+template <int X, int E>
+__device__ __forceinline__ void load_pair_rows(const float2 *__restrict__ g,
+                                               int Y, int ya, int tid,
+                                               float2 (&z)[E]) {
+  constexpr int NT = X / E;
+  const char *gb = reinterpret_cast<const char *>(g);
+  const unsigned lo =
+      ((((unsigned)(tid >> 2) * (unsigned)Y + (unsigned)ya) << 2) +
+       (unsigned)(tid & 3)) << 3;
+  const unsigned hi =
+      ((((unsigned)ya - (unsigned)((tid + 3) >> 2) * (unsigned)Y) << 2) +
+       (unsigned)((-tid) & 3)) << 3;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const unsigned ob =
+        m < E / 2 ? lo + (unsigned)(m * NT) * (unsigned)Y * 8u
+                  : hi + (unsigned)(X - m * NT) * (unsigned)Y * 8u;
+    const float2 A = *reinterpret_cast<const float2 *>(gb + ob);
+    const float2 B = *reinterpret_cast<const float2 *>(gb + ob + 32);
+    const float2 u = make_float2(A.x - B.y, A.y + B.x);
+    const float2 v = make_float2(A.x + B.y, B.x - A.y);
+    z[m] = m < E / 2 ? u : (m > E / 2 ? v : (tid == 0 ? u : v));
+  }
+}
+
 __device__ __forceinline__ float inv_norm(const uint32_t *tmax, int t,
                                           float mul) {
   if (tmax == nullptr) return 1.0f;
@@ -570,11 +606,15 @@ __device__ __forceinline__ void store_row_group(
   if (P == 1 || R0 + 2 * P > Y) {
     if (ya >= Y) return;
     const bool pair = yb < Y;  // ya is even: the pair is one aligned PT
+    // column n: base + n YB elements (32-bit offsets within one slab row
+    // block of ncols * YB elements)
+    ST *const base = slab + slab_index(k, t, ya, 0, T, NYB, ncols, YB);
+    const unsigned xo = (unsigned)tid * (unsigned)YB;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
       if (n >= ncols) continue;
-      ST *dst = slab + slab_index(k, t, ya, n, T, NYB, ncols, YB);
+      ST *dst = base + (xo + (unsigned)(m * NT) * (unsigned)YB);
       if (pair) {
         *reinterpret_cast<PT *>(dst) = out[m];
       } else {
@@ -634,12 +674,13 @@ __device__ __forceinline__ void store_row_group(
 // ramp_pair: one row pair (ya, yb) of one (tilt, defocus) spectrum g through
 // the four transforms: out[m] = rows (ya, yb) of column tid + m NT, times sc,
 // packed. ls: the pair's X-point LDS region; twx: this thread's X-point
-// twiddles; ct = c(tid).
+// twiddles; tw: the device table (c(n) = c(tid) exp(-i pi m / E) for
+// n = tid + m NT, both factors from its W^j entries).
 template <int X, int STORE, bool WAVE>
 __device__ __forceinline__ void ramp_pair(
     const float2 *g, int Y, int ya, int yb, float2 *ls, int tid,
     const typename Fft<X, default_e(X), 1, 1, WAVE>::Twiddles &twx,
-    float2 ct, const float2 *__restrict__ tw, float sc,
+    const float2 *__restrict__ tw, float sc,
     typename RowPair<STORE>::T (&out)[default_e(X)]) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
@@ -648,9 +689,13 @@ __device__ __forceinline__ void ramp_pair(
   // unnormalised inverse (powers of two: exact as the reference's product)
   constexpr float kRamp = 1.0f / ((float)X * (float)L);
   float2 z[E], e[E];
+  if (yb < Y) {
+    load_pair_rows<X, E>(g, Y, ya, tid, z);
+  } else {
 #pragma unroll
-  for (int m = 0; m < E; ++m)
-    z[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+    for (int m = 0; m < E; ++m)
+      z[m] = load_pair_spectrum(g, X, Y, ya, yb, tid + m * NT);
+  }
   // even bins: X r[2n] Z[n] = (fold(2n) / L) Z[n]
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -661,6 +706,7 @@ __device__ __forceinline__ void ramp_pair(
   Fft<X, E, 1, 1, WAVE>::run(e, ls, tid, twx);
   // odd bins: the rows, twiddled by c(n), forward, times r[2n + 1]
   Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
+  const float2 ct = tw[tid * (kTwN / L)];  // c(tid) = W^(tid kTwN / L)
 #pragma unroll
   for (int m = 0; m < E; ++m)
     z[m] = cmul(z[m], cmul(ct, tw[m * (kTwN / (2 * E))]));
@@ -674,9 +720,10 @@ __device__ __forceinline__ void ramp_pair(
   Fft<X, E, 1, 1, WAVE>::run(z, ls, tid, twx);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const float2 c = cmul(ct, tw[m * (kTwN / (2 * E))]);
-    const float2 o = cmul(z[m], make_float2(c.x, -c.y));
-    out[m] = RowPair<STORE>::pack((e[m].x + o.x) * sc, (e[m].y + o.y) * sc);
+    const float2 c = cmul(ct, tw[m * (kTwN / (2 * E))]);  // e + conj(c) z
+    const float ox = fmaf(c.x, z[m].x, fmaf(c.y, z[m].y, e[m].x));
+    const float oy = fmaf(c.x, z[m].y, fmaf(-c.y, z[m].x, e[m].y));
+    out[m] = RowPair<STORE>::pack(ox * sc, oy * sc);
   }
 }
 
@@ -728,39 +775,41 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
 
   typename Fft<X, E, 1, 1, WAVE>::Twiddles twx;  // both directions
   twx.load(tid, tw);
-  // c(tid + m NT) = c(tid) exp(-i pi m / E); both factors from the W^j table
-  constexpr int kHalf = kTwN / L;  // W^(kHalf n) = c(n)
-  const float2 ct = tw[tid * kHalf];
   const float sc =
       wscale[t] * (STORE == kF32 ? 1.0f : inv_norm(tmax, t, nrm_mul));
 
   PT out[E];
   if constexpr (IT == 1) {
     ramp_pair<X, STORE, WAVE>(g, Y, ya0 + 2 * p, ya0 + 2 * p + 1, ls, tid,
-                              twx, ct, tw, sc, out);
+                              twx, tw, sc, out);
     store_row_group<X, STORE, P>(out, s, p, tid, slab, k, t, R0, ny, X, T,
                                  NYB, YB);
   } else {
     const int Rs = R0 + 4 * p;  // this slot's four rows (band-relative)
     const int ys = y0 + Rs;
     PT *sp = stash + p * X;
-    ramp_pair<X, STORE, WAVE>(g, Y, ys, ys + 1, ls, tid, twx, ct, tw, sc, out);
+    ramp_pair<X, STORE, WAVE>(g, Y, ys, ys + 1, ls, tid, twx, tw, sc, out);
 #pragma unroll
     for (int m = 0; m < E; ++m) sp[tid + m * NT] = out[m];
-    // The second pair recomputes its per-column factors (c(n), ramp)
+    // The second pair recomputes its per-column values (offsets, ramp)
     // instead of keeping the first pair's: held across its transforms they
-    // would take 64 more registers.
+    // would take more registers than the cap leaves.
     int tid2 = tid;
-    float2 ct2 = ct;
-    asm volatile("" : "+v"(tid2), "+v"(ct2)::"memory");
-    ramp_pair<X, STORE, WAVE>(g, Y, ys + 2, ys + 3, ls, tid2, twx, ct2, tw, sc,
+    asm volatile("" : "+v"(tid2)::"memory");
+    ramp_pair<X, STORE, WAVE>(g, Y, ys + 2, ys + 3, ls, tid2, twx, tw, sc,
                               out);
     if (Rs >= ny) return;
     const bool whole = Rs + 4 <= ny;  // Rs % 4 == 0: the run is 8-byte aligned
+    // column n of the four rows: base + n YB elements (32-bit byte offsets
+    // within one slab row block of X * YB elements)
+    char *base = reinterpret_cast<char *>(
+        slab + slab_index(k, t, Rs, 0, T, NYB, X, YB));
+    const unsigned xb = (unsigned)tid * (unsigned)YB * (unsigned)sizeof(ST);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
-      ST *dst = slab + slab_index(k, t, Rs, n, T, NYB, X, YB);
+      ST *dst = reinterpret_cast<ST *>(
+          base + xb + (unsigned)(m * NT) * (unsigned)YB * (unsigned)sizeof(ST));
       const uint2 w = make_uint2(sp[n], out[m]);
       if (whole) {
         *reinterpret_cast<uint2 *>(dst) = w;
