@@ -650,7 +650,7 @@ def bp_roofline(w, entry, counters):
     workload). What binds it is instruction issue: VALU instructions per
     SIMD per clock (counted instructions / this run's launch time) against
     the FP32 issue rate probe kernels reach, and LDS bytes against the
-    measured ds_read_b128 peak. The SURVEY's gather model (two taps of s
+    measured ds_read_b128 peak. The gather model (two taps of s
     bytes per update) is kept as a labelled figure: staging makes its taps
     LDS reads, so it exceeds the HBM peak."""
     X, Y, Z, T = w["X"], w["Y"], w["Z"], w["T"]
