@@ -232,21 +232,30 @@ struct Staging {
 };
 
 // Window of one tilt over the tile's active voxels [x0, x1] x [z0, z1]:
-// smallest i0 and smallest k over the four corners.
+// smallest i0 and smallest k over the four corners, with kOneSlab set in .y
+// when the largest k over the corners is the same: slab_of is monotone in x
+// and in z like p, so every voxel of the tile then takes slab k.
+constexpr int kOneSlab = 1 << 30;
+constexpr int kSlabMask = kOneSlab - 1;
+
 template <bool EXACT, bool MULTI>
 __device__ __forceinline__ int2 tile_window(float4 cf, float x0, float x1,
                                             float z0, float z1, float dscale,
                                             float doffset, int ndf) {
-  int w = 0x7fffffff, k = 0x7fffffff;
+  int w = 0x7fffffff, k = 0x7fffffff, kmax = -1;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const float x = (c & 1) ? x1 : x0;
     const float z = (c & 2) ? z1 : z0;
     float tt;
     w = min(w, tap_index<EXACT>(x, z, cf, tt));
-    if constexpr (MULTI) k = min(k, slab_of(x, z, cf, dscale, doffset, ndf));
+    if constexpr (MULTI) {
+      const int kc = slab_of(x, z, cf, dscale, doffset, ndf);
+      k = min(k, kc);
+      kmax = max(kmax, kc);
+    }
   }
-  return make_int2(w, MULTI ? k : 0);
+  return make_int2(w, MULTI ? (k | (kmax == k ? kOneSlab : 0)) : 0);
 }
 
 // win[tile * Tpad + t], tile = bx * nbz + bz (the kernels' order).
@@ -491,8 +500,9 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
       const int j = q / KS, sl = q % KS;
       const int t = tb + j;
       const int2 w = kload(tw, t);
-      const int k = MULTI ? w.y + sl : 0;
-      const bool ok = t < T && k < ndf;
+      const int k = MULTI ? (w.y & kSlabMask) + sl : 0;
+      // a tile within one slab at this tilt reads only its first run
+      const bool ok = t < T && k < ndf && (sl == 0 || !(w.y & kOneSlab));
       const ST *src = yblock + (size_t)t * tstride + (size_t)k * kstride;
       // One range-checked raw buffer per run: its N positions of YB values.
       // A load at an offset outside [0, N * YB * sizeof(T)) returns zeros
@@ -548,8 +558,12 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
         const int2 w = kload(tw, t);
         int ks = 0;
         if constexpr (MULTI) {
-          ks = slab_of(x, z, cf, dscale, doffset, ndf) - w.y;
-          ks = min(max(ks, 0), KS - 1);
+          // A tile within one slab at this tilt needs no per-voxel slab
+          // choice (w is the workgroup's own entry: the branch is uniform).
+          if (!(w.y & kOneSlab)) {
+            ks = slab_of(x, z, cf, dscale, doffset, ndf) - (w.y & kSlabMask);
+            ks = min(max(ks, 0), KS - 1);
+          }
         }
         float tt;
         const int i0 = tap_index<kExact>(x, z, cf, tt);

@@ -25,7 +25,8 @@ struct BpArgs {
   //   reduced: c, s, offset - 1/2, scale * normalisation (bp_coefficients)
   const float4 *coef;
   const int2 *win;           // staged kernel: per (tile, tilt) window base and
-                             // first slab (bp_windows)
+                             // first slab, bit 30 set when the tile lies in
+                             // that one slab (bp_windows)
   float dscale, doffset;     // slab selection (reconstructor.cpp:29-32)
   int ndf, N, T, NYB, W, Z, Y;
   int yb0, yb1;              // row blocks [yb0, yb1) to reconstruct
