@@ -185,10 +185,10 @@ using gx::filter_length;
 using gx::ramp_table;
 using gx::tilt_geometry;
 
-// The power-of-two correction kernels take 8 <= X <= 4096 and 4 <= Y <=
-// 4096; other sizes use Bluestein transforms of length filter_length(n).
-bool rows_pow2(size_t X) { return is_pow2(X) && X >= 8 && X <= 4096; }
-bool cols_pow2(size_t Y) { return is_pow2(Y) && Y >= 4 && Y <= 4096; }
+// The power-of-two correction kernels take 8 <= X <= 8192 and 4 <= Y <=
+// 8192; other sizes use Bluestein transforms of length filter_length(n).
+bool rows_pow2(size_t X) { return is_pow2(X) && X >= 8 && X <= 8192; }
+bool cols_pow2(size_t Y) { return is_pow2(Y) && Y >= 4 && Y <= 8192; }
 
 // In-place radix-2 FFT (sign -1) of n = 2^k complex doubles.
 void fft_f64(std::vector<std::complex<double>> &a) {
@@ -499,9 +499,9 @@ int gx::check_recon_cfg(const gx_recon_cfg *c) {
   GX_CHECK(c->pixel_size > 0);
   GX_CHECK(c->min_defocus <= c->max_defocus);
   GX_CHECK(c->angles != nullptr);
-  if (filter_length(c->num_pixels) > 8192)
+  if (filter_length(c->num_pixels) > 16384)
     return fail(GX_EUNSUPPORTED,
-                "num_pixels %zu exceeds the gfx950 filter limit (4096)",
+                "num_pixels %zu exceeds the gfx950 filter limit (8192)",
                 c->num_pixels);
   return GX_OK;
 }
@@ -588,8 +588,10 @@ extern "C" int gx_filter(const float *rows, float *out, size_t nrows,
                          size_t num_pixels, int gpu_index) {
   GX_CHECK(rows != nullptr && out != nullptr);
   GX_CHECK(num_pixels > 0 && nrows > 0);
-  if (filter_length(num_pixels) > 8192)
-    return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", num_pixels);
+  if (filter_length(num_pixels) > 16384)
+    return fail(GX_EUNSUPPORTED,
+                "num_pixels %zu exceeds the gfx950 filter limit (8192)",
+                num_pixels);
   int rc = select_device(gpu_index);
   if (rc) return rc;
   const size_t n = nrows * num_pixels;
@@ -639,10 +641,12 @@ extern "C" int gx_recon(const gx_recon_cfg *cfg, const float *sino,
   return GX_OK;
 }
 
+// Any size from 1 to 8192 in each dimension (Bluestein transforms of up to
+// 16384 points for the sizes the power-of-two kernels do not take).
 static int check_corr_sizes(size_t X, size_t Y) {
-  if (X < 1 || X > 4096 || Y < 1 || Y > 4096)
+  if (X < 1 || X > 8192 || Y < 1 || Y > 8192)
     return fail(GX_EUNSUPPORTED,
-                "CTF correction on gfx950 takes 1 <= xsize, ysize <= 4096 "
+                "CTF correction on gfx950 takes 1 <= xsize, ysize <= 8192 "
                 "(got %zu x %zu)", X, Y);
   return GX_OK;
 }
@@ -652,7 +656,7 @@ static int check_pow2_corr_sizes(size_t X, size_t Y) {
   if (!rows_pow2(X) || !cols_pow2(Y))
     return fail(GX_EUNSUPPORTED,
                 "synthetic CTF data needs power-of-two sizes, 8 <= xsize <= "
-                "4096 and 4 <= ysize <= 4096 (got %zu x %zu)", X, Y);
+                "8192 and 4 <= ysize <= 8192 (got %zu x %zu)", X, Y);
   return GX_OK;
 }
 
@@ -883,8 +887,9 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
   if (ps->ctf) {
     int rc = check_corr_sizes(X, Y);
     if (rc) return rc;
-  } else if (filter_length(X) > 8192) {
-    return fail(GX_EUNSUPPORTED, "num_pixels %zu exceeds 4096", X);
+  } else if (filter_length(X) > 16384) {
+    return fail(GX_EUNSUPPORTED,
+                "num_pixels %zu exceeds the gfx950 filter limit (8192)", X);
   }
   ps->YB = d->storage == GX_STORE_F32 ? 16 : 32;
   // A banded plan holds the filtered slabs of band_rows detector rows at a
@@ -914,7 +919,7 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
       const size_t budget = (size_t)4 << 30, cap = (size_t)16 << 30;
       chunk = std::max<size_t>(1, budget / per_tilt);
       if (Y >= 2048 && cols_pow2(Y)) {
-        const size_t wg = (Y == 4096 ? 2 : 1) * (size_t)ps->XB;
+        const size_t wg = (Y >= 4096 ? 2 : 1) * (size_t)ps->XB;
         const size_t want = (16 * 512 + wg - 1) / wg;
         chunk = std::max(chunk, std::min(want, cap / per_tilt));
       }

@@ -605,20 +605,27 @@ __device__ __forceinline__ void store_row_group(
   const int ya = R0 + 2 * p, yb = ya + 1;
   if (P == 1 || R0 + 2 * P > Y) {
     if (ya >= Y) return;
-    const bool pair = yb < Y;  // ya is even: the pair is one aligned PT
+    // ya is even: with an even row blocking the pair is one aligned PT;
+    // with YB = 1 (one row per block) row yb lies a whole row further
+    const bool pair = yb < Y && YB % 2 == 0;
     // column n: base + n YB elements (32-bit offsets within one slab row
     // block of ncols * YB elements)
     ST *const base = slab + slab_index(k, t, ya, 0, T, NYB, ncols, YB);
+    ST *const base_b =
+        yb < Y ? slab + slab_index(k, t, yb, 0, T, NYB, ncols, YB) : base;
     const unsigned xo = (unsigned)tid * (unsigned)YB;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int n = tid + m * NT;
       if (n >= ncols) continue;
-      ST *dst = base + (xo + (unsigned)(m * NT) * (unsigned)YB);
+      const unsigned o = xo + (unsigned)(m * NT) * (unsigned)YB;
+      ST *dst = base + o;
       if (pair) {
         *reinterpret_cast<PT *>(dst) = out[m];
       } else {
-        dst[0] = reinterpret_cast<const ST *>(&out[m])[0];
+        const ST *v = reinterpret_cast<const ST *>(&out[m]);
+        dst[0] = v[0];
+        if (yb < Y) base_b[o] = v[1];
       }
     }
     return;
@@ -739,7 +746,7 @@ __device__ __forceinline__ void ramp_pair(
 // as one 8-byte write.
 template <int X, int STORE, int P, int IT>
 __global__ __launch_bounds__(P *(X / default_e(X)))
-__attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
+__attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_rows_c2r_ramp(
     const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
@@ -823,9 +830,11 @@ __attribute__((amdgpu_waves_per_eu(3))) void k_rows_c2r_ramp(
   }
 }
 
+// 8192-point halves: a 512-thread workgroup with 68 KB of LDS, so at most
+// two waves per SIMD share a CU anyway and the kernel may take 256 VGPRs.
 template <int X, int STORE, int P>
 __global__ __launch_bounds__(P *(X / default_e(X)))
-__attribute__((amdgpu_waves_per_eu(3))) void k_filter_ramp(
+__attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_filter_ramp(
     const float *__restrict__ src, size_t st, size_t sk, size_t sy, int N,
     int Y, int T, int ndf, const float *__restrict__ ramp,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
@@ -1133,20 +1142,26 @@ __device__ __forceinline__ void bluestein(
   }
 }
 
+// Bluestein row passes: P row pairs per workgroup, one M-point slot each
+// (P = 2; P = 1 at M = 16384, whose slot is a whole 1024-thread workgroup
+// with 139 KB of LDS).
+constexpr int bs_row_slots(int M) { return M >= 16384 ? 1 : 2; }
+
 // Pass 1 for any X: rows (ya, yb) as one complex row, forward X-point DFT,
 // then the Hermitian split into the two half spectra (as k_rows_r2c).
-template <int M>
-__global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_r2c_bs(
+template <int M, int P = bs_row_slots(M)>
+__global__ __launch_bounds__(P * (M / default_e(M))) void k_rows_r2c_bs(
     const float *__restrict__ img, size_t tstride, int X, int Y, int XB,
     float2 *__restrict__ F, uint32_t *__restrict__ tmax, int t0,
     const float2 *__restrict__ tw, BluesteinTable bx) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
-  __shared__ float2 s[2][fft_lds_len(M, E)];
+  constexpr int RB = 2 * P;  // rows per workgroup
+  __shared__ float2 s[P][fft_lds_len(M, E)];
   __shared__ float red[16];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int t = blockIdx.y;
-  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const int ya = blockIdx.x * RB + 2 * p, yb = ya + 1;
   const float *im = img + (size_t)t * tstride;
   typename Fft<M, E, -1, 1>::Twiddles twm;
   twm.load(tid, tw);
@@ -1168,13 +1183,13 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_r2c_bs(
   }
   lds_barrier();
   if (tmax != nullptr) {
-    amax = block_max<2 * NT>(amax, red);
+    amax = block_max<P * NT>(amax, red);
     if (threadIdx.x == 0) atomicMax(&tmax[t0 + t], __float_as_uint(amax));
   }
   const int ncol = XB * CB;
-  for (int e = threadIdx.x; e < 4 * ncol; e += 2 * NT) {
-    const int ci = e % CB, r = (e / CB) % 4, cb = e / (4 * CB);
-    const int kx = cb * CB + ci, y = blockIdx.x * 4 + r;
+  for (int e = threadIdx.x; e < RB * ncol; e += P * NT) {
+    const int ci = e % CB, r = (e / CB) % RB, cb = e / (RB * CB);
+    const int kx = cb * CB + ci, y = blockIdx.x * RB + r;
     if (y >= Y) continue;
     float2 val = make_float2(0.0f, 0.0f);
     if (kx <= X / 2) {
@@ -1239,16 +1254,16 @@ __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
 }
 
 // Pass 3 for any X, unfiltered (corr()): out[t][k][y][x].
-template <int M>
-__global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_c2r_bs(
+template <int M, int P = bs_row_slots(M)>
+__global__ __launch_bounds__(P * (M / default_e(M))) void k_rows_c2r_bs(
     const float2 *__restrict__ G, int X, int Y, int XB, int ndf,
     float *__restrict__ out, const float2 *__restrict__ tw, BluesteinTable bx) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
-  __shared__ float2 s[2][fft_lds_len(M, E)];
+  __shared__ float2 s[P][fft_lds_len(M, E)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z;
-  const int ya = blockIdx.x * 4 + 2 * p, yb = ya + 1;
+  const int ya = blockIdx.x * (2 * P) + 2 * p, yb = ya + 1;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
   typename Fft<M, E, -1, 1>::Twiddles twm;
   twm.load(tid, tw);
@@ -1274,8 +1289,8 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void k_rows_c2r_bs(
 // Pass 3 for any X, fused with the ramp filter: the corrected rows (X
 // samples, unnormalised inverse) are zero-padded to M >= 2X - 1 and filtered
 // with the truncated-kernel ramp table, as k_filter_rows does for real rows.
-template <int M, int STORE>
-__global__ __launch_bounds__(2 * (M / default_e(M))) void
+template <int M, int STORE, int P = bs_row_slots(M)>
+__global__ __launch_bounds__(P * (M / default_e(M))) void
     k_rows_c2r_filter_bs(const float2 *__restrict__ G, int X, int Y, int XB,
                          int ndf, int t0, int T,
                          const float *__restrict__ ramp,
@@ -1286,11 +1301,11 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void
                          BluesteinTable bx, int y0, int ny) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
-  __shared__ float2 s[2][fft_lds_len(M, E)];
+  __shared__ float2 s[P][fft_lds_len(M, E)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
   // band-relative rows ra, rb of the slab; spectrum rows y0 + ra, y0 + rb
-  const int ra = blockIdx.x * 4 + 2 * p, rb = ra + 1;
+  const int ra = blockIdx.x * (2 * P) + 2 * p, rb = ra + 1;
   const int ya = y0 + ra, yb = ya + 1;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
   typename Fft<M, E, -1, 1>::Twiddles twm;
@@ -1327,7 +1342,8 @@ __global__ __launch_bounds__(2 * (M / default_e(M))) void
 // ------------------------------------------------------------ launchers ---
 
 #define GX_POW2_CASES(M) \
-  M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096)
+  M(4) M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096) \
+  M(8192)
 
 hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
                            int XB, int ntilts, float2 *F, uint32_t *tmax,
@@ -1340,7 +1356,8 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
   switch (X) {
 #define C_(n)                                                                  \
   case n: {                                                                    \
-    constexpr int rp = n == 1024 ? 8 : (n == 512 || n == 2048) ? 4 : 2;        \
+    constexpr int rp =                                                         \
+        n == 1024 ? 8 : (n == 512 || n == 2048) ? 4 : n == 8192 ? 1 : 2;       \
     const dim3 grid((Y + 2 * rp - 1) / (2 * rp), ntilts);                      \
     hipLaunchKernelGGL((k_rows_r2c<n, rp>), grid,                              \
                        dim3(rp * (n / default_e(n))), 0, s, img, tstride, Y,   \
@@ -1374,6 +1391,13 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
     C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
     C_(2048) C_(4096)
 #undef C_
+    case 8192:
+      // two columns per 1024-thread workgroup (139 KB of LDS), the two
+      // halves of a column block on one XCD
+      hipLaunchKernelGGL((k_cols<8192, M, 2>), dim3(2 * XB * ntilts),
+                         dim3(2 * (8192 / default_e(8192))), 0, s, F, mask, G,
+                         XB, ndf, tw, ylo, yhi);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -1433,7 +1457,8 @@ static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
 // workgroups that wait at each other's barriers stay small.
 inline int c2r_ramp_slots(int X, int YB) {
   const int cap = X <= 1024 ? 4 : (X == 2048 ? 2 : 1);
-  return YB / 2 < cap ? YB / 2 : cap;
+  const int p = YB / 2 < cap ? YB / 2 : cap;
+  return p < 1 ? 1 : p;  // YB = 1: one slot, rows stored one by one
 }
 
 template <int X, int STORE, int P, int IT = 1>
@@ -1455,8 +1480,8 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw, int y0,
                                     int ny) {
-  // two row pairs per slot at 4096 points with 16-bit storage
-  const int rb = X == 4096 && STORE != kF32 ? 4 * P : 2 * P;
+  // two row pairs per slot at 4096 and 8192 points with 16-bit storage
+  const int rb = X >= 4096 && STORE != kF32 ? 4 * P : 2 * P;
   const size_t nb = (size_t)((ny + rb - 1) / rb) * ndf * ntilts;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
@@ -1477,6 +1502,15 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
         break;
       } else if constexpr (P == 1) {
         launch_c2r_ramp_x<4096, STORE, 1>(A_);
+        break;
+      }
+      return hipErrorInvalidValue;
+    case 8192:
+      if constexpr (P == 1 && STORE != kF32) {
+        launch_c2r_ramp_x<8192, STORE, 1, 2>(A_);
+        break;
+      } else if constexpr (P == 1) {
+        launch_c2r_ramp_x<8192, STORE, 1>(A_);
         break;
       }
       return hipErrorInvalidValue;
@@ -1663,6 +1697,12 @@ static hipError_t launch_filter_ramp_p(const float *src, size_t st, size_t sk,
         break;
       }
       return hipErrorInvalidValue;
+    case 8192:
+      if constexpr (P <= 1) {
+        launch_filter_ramp_x<8192, STORE, P>(A_);
+        break;
+      }
+      return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
@@ -1699,8 +1739,9 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
     tw = twiddles(&err);
     if (err != hipSuccess) return err;
   }
-  // 512- to 8192-point filters: four half-length transforms per row pair
-  if (L >= 512 && L <= 8192 && YB >= 8) {
+  // 512- to 16384-point filters: four half-length transforms per row pair
+  // (16384 points, rows of 4097 to 8192: at any row blocking, one slot)
+  if (L >= 512 && (L == 16384 || (L <= 8192 && YB >= 8))) {
 #define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
     switch (store) {
       case kF32: return launch_filter_ramp_t<kF32>(A_);
@@ -1759,7 +1800,8 @@ hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
 namespace gx {
 
 #define GX_BS_CASES(M) \
-  M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096) M(8192)
+  M(8) M(16) M(32) M(64) M(128) M(256) M(512) M(1024) M(2048) M(4096) M(8192) \
+  M(16384)
 
 hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
                               int XB, int ntilts, float2 *F, uint32_t *tmax,
@@ -1768,13 +1810,15 @@ hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  dim3 grid((Y + 3) / 4, ntilts);
   switch (M) {
 #define C_(n)                                                                \
-  case n:                                                                    \
-    hipLaunchKernelGGL(k_rows_r2c_bs<n>, grid, dim3(2 * (n / default_e(n))), \
+  case n: {                                                                  \
+    constexpr int P = bs_row_slots(n);                                       \
+    const dim3 grid((Y + 2 * P - 1) / (2 * P), ntilts);                      \
+    hipLaunchKernelGGL(k_rows_r2c_bs<n>, grid, dim3(P * (n / default_e(n))), \
                        0, s, img, tstride, X, Y, XB, F, tmax, t0, tw, bx);   \
-    break;
+    break;                                                                   \
+  }
     GX_BS_CASES(C_)
 #undef C_
     default:
@@ -1793,7 +1837,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
   if (err != hipSuccess) return err;
   // CB columns of a block per workgroup up to 2048-point transforms, one
   // column at 4096 and 8192 points (1024-thread workgroups would cap the
-  // registers at 128 and spill)
+  // registers at 128 and spill) and at 16384 (a whole 1024-thread workgroup)
   switch (M) {
 #define C_(n)                                                                 \
   case n:                                                                     \
@@ -1809,7 +1853,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                        dim3(n / default_e(n)), 0, s, F, mask, G, Y, XB, ndf,  \
                        tw, by, ylo, yhi);                                     \
     break;
-    C_(4096) C_(8192)
+    C_(4096) C_(8192) C_(16384)
 #undef C_
     default:
       return hipErrorInvalidValue;
@@ -1823,13 +1867,15 @@ hipError_t launch_rows_c2r_bs(const float2 *G, int X, int Y, int XB, int ndf,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  dim3 grid((Y + 3) / 4, ndf, ntilts);
   switch (M) {
 #define C_(n)                                                                \
-  case n:                                                                    \
-    hipLaunchKernelGGL(k_rows_c2r_bs<n>, grid, dim3(2 * (n / default_e(n))), \
+  case n: {                                                                  \
+    constexpr int P = bs_row_slots(n);                                       \
+    const dim3 grid((Y + 2 * P - 1) / (2 * P), ndf, ntilts);                 \
+    hipLaunchKernelGGL(k_rows_c2r_bs<n>, grid, dim3(P * (n / default_e(n))), \
                        0, s, G, X, Y, XB, ndf, out, tw, bx);                 \
-    break;
+    break;                                                                   \
+  }
     GX_BS_CASES(C_)
 #undef C_
     default:
@@ -1849,15 +1895,17 @@ static hipError_t launch_c2r_filter_bs_t(const float2 *G, int X, int Y,
                                          hipStream_t s, const float2 *tw,
                                          int y0, int ny) {
   using ST = typename StoreT<STORE>::T;
-  dim3 grid((ny + 3) / 4, ndf, ntilts);
   switch (M) {
 #define C_(n)                                                                  \
-  case n:                                                                      \
+  case n: {                                                                    \
+    constexpr int P = bs_row_slots(n);                                         \
+    const dim3 grid((ny + 2 * P - 1) / (2 * P), ndf, ntilts);                  \
     hipLaunchKernelGGL((k_rows_c2r_filter_bs<n, STORE>), grid,                 \
-                       dim3(2 * (n / default_e(n))), 0, s, G, X, Y, XB, ndf,   \
+                       dim3(P * (n / default_e(n))), 0, s, G, X, Y, XB, ndf,   \
                        t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
                        tw, bx, y0, ny);                                        \
-    break;
+    break;                                                                     \
+  }
     GX_BS_CASES(C_)
 #undef C_
     default:

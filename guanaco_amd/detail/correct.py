@@ -18,9 +18,11 @@ __all__ = ["correct_projections", "correct_file", "get_defocus_steps",
            "get_corrected_array", "get_ctf", "get_ctf_array"]
 
 
-# Detector sizes of the gfx950 kernels: the correction's 2D transforms and
-# the ramp filter's 2N-point transforms take up to 4096 rows and columns.
-MAX_DETECTOR = 4096
+# Detector sizes of the gfx950 kernels: the ramp filter takes rows of up to
+# 8192 pixels (2N-point filters up to 16384 points), the 2D CTF correction
+# up to 8192 rows and columns (power-of-two transforms, Bluestein transforms
+# of up to 16384 points for other sizes).
+MAX_DETECTOR = 8192
 
 
 def check_detector_size(width, height=None):

@@ -4,8 +4,9 @@
  *
  * Plain C: pointers, sizes and status codes only. Every entry point returns
  * GX_OK (0) or a non-zero GX_E* code; gx_last_error() then describes the
- * failure (thread-local). All entry points are thread-safe; none uses device
- * globals, so calls on different streams may overlap.
+ * failure (thread-local). All entry points are thread-safe; none but
+ * gx_recon_noalloc uses device globals, so calls on different streams may
+ * overlap (gx_recon_noalloc calls on one GPU may not).
  *
  * The reference's GPU path is reached through pybind11 (guanaco_ext.recon /
  * guanaco_ext.corr) and C++ classes; the entries below replace those classes
@@ -129,8 +130,9 @@ int gx_filter(const float *rows, float *out, size_t nrows, size_t num_pixels,
  * guanaco::correct (correct.h:77-95): image (ysize x xsize), ctf complex64
  * interleaved (num_ctf x ysize x xsize), rec (num_ctf x ysize x xsize)
  * overwritten with Re(IFFT2_unnorm(FFT2(image) * -sign(Im ctf))).
- * Any xsize and ysize in [1, 4096]: powers of two (xsize >= 8, ysize >= 4)
- * take the power-of-two transforms, other sizes Bluestein transforms. */
+ * Any xsize and ysize in [1, 8192]: powers of two (xsize >= 8, ysize >= 4)
+ * take the power-of-two transforms, other sizes Bluestein transforms (of up
+ * to 16384 points). */
 int gx_corr(const float *image, const float *ctf, float *rec, size_t xsize,
             size_t ysize, size_t num_ctf, int gpu_index);
 

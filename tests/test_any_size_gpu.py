@@ -25,6 +25,9 @@ def rel_l2(a, b):
 @pytest.mark.parametrize("X,Y,ndf", [
     (30, 20, 1), (33, 17, 3), (100, 64, 2), (64, 48, 2), (1, 5, 1),
     (4, 2, 1), (600, 250, 2),
+    # beyond 4096: 16384-point Bluestein rows (one slot per workgroup) and
+    # columns
+    (4100, 6, 1), (10, 5000, 2),
 ])
 def test_corr_any_size_matches_oracle(capi, X, Y, ndf):
     rng = numpy.random.default_rng(X * 1000 + Y)
@@ -53,7 +56,8 @@ def test_corr_any_size_closed_form(capi):
     assert rel_l2(got, -(X * Y) * img.astype(numpy.float64)) < 1e-6
 
 
-@pytest.mark.parametrize("X,Y,ndf", [(96, 40, 3), (100, 50, 1), (64, 30, 2)])
+@pytest.mark.parametrize("X,Y,ndf", [(96, 40, 3), (100, 50, 1), (64, 30, 2),
+                                     (5000, 6, 2)])
 def test_fused_volume_path_any_size(capi, X, Y, ndf):
     T = 9
     rng = numpy.random.default_rng(X + Y + ndf)

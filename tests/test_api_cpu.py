@@ -173,16 +173,20 @@ def test_reconstruct_volume_checks_array_sizes():
                                     wavelength=0.02, Cs=2.7e7)
 
 
-def test_detectors_above_4096_are_refused_before_any_work(tmp_path):
-    # the gfx950 kernels take up to 4096 rows and columns; larger detectors
+def test_detectors_above_8192_are_refused_before_any_work(tmp_path):
+    # the gfx950 kernels take up to 8192 rows and columns; larger detectors
     # raise a RuntimeError naming the limit (no GPU needed to refuse them)
     from guanaco_amd.detail.correct import check_detector_size
-    check_detector_size(4096, 4096)
-    with pytest.raises(RuntimeError, match="width 5760 exceeds the 4096"):
-        check_detector_size(5760, 4092)
-    with pytest.raises(RuntimeError, match="height 5000"):
-        check_detector_size(1000, 5000)
-    proj = numpy.zeros((3, 8, 4100), numpy.float32)
-    with pytest.raises(RuntimeError, match="4096"):
+    check_detector_size(8192, 8192)
+    check_detector_size(5760, 4092)
+    check_detector_size(8192)
+    with pytest.raises(RuntimeError, match="width 8193 exceeds the 8192"):
+        check_detector_size(8193, 4092)
+    with pytest.raises(RuntimeError, match="height 9000"):
+        check_detector_size(1000, 9000)
+    with pytest.raises(RuntimeError, match="width 8193"):
+        check_detector_size(8193)
+    proj = numpy.zeros((3, 8, 8200), numpy.float32)
+    with pytest.raises(RuntimeError, match="8192"):
         G.reconstruct_projections(proj, numpy.zeros(3, numpy.float32),
                                   defocus=20000.0, num_defocus=2)

@@ -83,15 +83,15 @@ def test_recon_rejects_invalid_config(kw, text):
 
 
 def test_corr_rejects_unsupported_sizes():
-    # any size up to 4096 x 4096 is taken (power-of-two kernels or Bluestein
+    # any size up to 8192 x 8192 is taken (power-of-two kernels or Bluestein
     # transforms); wider rows are refused before any device work
-    img = numpy.zeros((2, 5000), numpy.float32)
-    ctf = numpy.zeros((2, 5000), numpy.complex64)
-    out = numpy.zeros((2, 5000), numpy.float32)
+    img = numpy.zeros((2, 8200), numpy.float32)
+    ctf = numpy.zeros((2, 8200), numpy.complex64)
+    out = numpy.zeros((2, 8200), numpy.float32)
     with pytest.raises(_capi.GxError) as e:
         _capi.call("gx_corr", _capi.f32p(img),
                    ctf.view(numpy.float32).ctypes.data_as(_capi._f32p),
-                   _capi.f32p(out), 5000, 2, 1, -1)
+                   _capi.f32p(out), 8200, 2, 1, -1)
     assert e.value.code == _capi.GX_EUNSUPPORTED
 
 
@@ -115,8 +115,11 @@ def test_plan_footprint_is_host_only_arithmetic():
     odd = _capi.plan_bytes(num_pixels=1000, num_rows=1000, depth=256,
                            angles=angles, defocus=[1e4], storage="f16")
     assert odd > 121 * 1000 * 1000 * 2
+    wide = _capi.plan_bytes(num_pixels=5000, num_rows=1024, depth=256,
+                            angles=angles, defocus=[1e4], storage="f16")
+    assert wide > 121 * 1024 * 5000 * 2
     with pytest.raises(_capi.GxError):
-        _capi.plan_bytes(num_pixels=5000, num_rows=1024, depth=256,
+        _capi.plan_bytes(num_pixels=8200, num_rows=1024, depth=256,
                          angles=angles, defocus=[1e4])
 
 

@@ -30,7 +30,8 @@ def tilts(T, span=60.0):
 
 # ------------------------------------------------------------- filter ------
 
-@pytest.mark.parametrize("N", [8, 13, 64, 100, 256, 1024, 1500, 4096])
+@pytest.mark.parametrize("N", [8, 13, 64, 100, 256, 1024, 1500, 4096, 5760,
+                               8192])
 def test_filter_matches_oracle(capi, N):
     rng = numpy.random.default_rng(N)
     rows = rng.standard_normal((5, N)).astype(numpy.float32)
@@ -67,7 +68,10 @@ def test_project_bitexact_vs_oracle(capi, case):
     numpy.testing.assert_array_equal(got, expect)
 
 
-@pytest.mark.parametrize("case", BP_CASES[1:4])
+@pytest.mark.parametrize("case", BP_CASES[1:4] + [
+    # rows wider than 4096: 16384-point filters, one row per row block
+    (5760, 5, 2, 24, 5760, 2880.0, (-40.0, 40.0), 1.0),
+    (8192, 3, 1, 16, 8192, 4096.0, (0.0, 0.0), 1.0)])
 def test_recon_matches_oracle(capi, case):
     N, T, ndf, H, W, centre, (mn, mx), ps = case
     rng = numpy.random.default_rng(7 + N)
@@ -99,6 +103,9 @@ CORR_CASES = [
     # defoci through the thread-order flip mask
     (2048, 32, 1.5, [30000.0, 30500.0], 0.0, 0.0, 0.1),
     (4096, 16, 1.0, [20000.0, 21000.0, 22000.0], 300.0, 0.3, 0.1),
+    # 8192 points: two-column workgroups of 1024 threads, and 8192-point rows
+    (8192, 16, 1.0, [20000.0, 20500.0], 0.0, 0.0, 0.1),
+    (8, 8192, 1.0, [25000.0], 200.0, 0.2, 0.1),
 ]
 
 
@@ -236,6 +243,10 @@ RAMP_CTF_CASES = [
     # workgroup at 1024 points, 8 at 2048)
     (1024, 21, 3, 2, 4, 2.0, 22000.0),
     (2048, 11, 2, 1, 4, 1.5, 28000.0),
+    # 8192-point rows (16384-point ramp, two row pairs per slot with 16-bit
+    # storage) and 8192-point columns
+    (8192, 6, 2, 2, 4, 1.0, 25000.0),
+    (32, 8192, 2, 2, 4, 1.0, 25000.0),
 ]
 
 
@@ -264,6 +275,8 @@ RAMP_PLAIN_CASES = [
     (1024, 8, 5, 8),
     (2048, 5, 3, 4),
     (4096, 3, 3, 4),
+    (5000, 3, 3, 4),
+    (8192, 3, 2, 4),
 ]
 
 
