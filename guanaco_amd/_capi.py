@@ -185,6 +185,8 @@ SIGNATURES = {
     "gx_volume_plan_set_backprojector": [_vp, _int],
     "gx_volume_plan_get_backprojector": [_vp, ctypes.POINTER(_int),
                                          ctypes.POINTER(_int)],
+    "gx_volume_plan_slab_range": [_vp, ctypes.POINTER(_int),
+                                  ctypes.POINTER(_int)],
     "gx_volume_slabs": [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_sz),
                         ctypes.POINTER(_int)],
     "gx_volume_plan_autotune": [_vp, _vp, _vp, _vp,
@@ -440,6 +442,7 @@ class VolumePlan:
         width = num_pixels if width is None else width
         self.shape = (num_rows, depth, width)
         self.storage = storage
+        self.num_angles = len(angles)
         p = _vp()
         call("gx_volume_plan_create", ctypes.byref(self.desc), ctypes.byref(p))
         self.handle = p.value
@@ -521,6 +524,18 @@ class VolumePlan:
                     {"backprojector": names[c.kind], "rows": c.rows,
                      "filter_ms": round(c.filter_ms, 4),
                      "bp_ms": round(c.bp_ms, 4)} for c in r.cand[:r.n]]}
+
+    def slab_range(self):
+        """(first, last): per tilt, the defocus slabs the filter computes
+        (those some voxel of the volume reads)."""
+        import numpy
+        T = self.num_angles
+        first = numpy.zeros(T, numpy.int32)
+        last = numpy.zeros(T, numpy.int32)
+        call("gx_volume_plan_slab_range", self.handle,
+             first.ctypes.data_as(ctypes.POINTER(_int)),
+             last.ctypes.data_as(ctypes.POINTER(_int)))
+        return first, last
 
     def slabs(self):
         p = _vp()

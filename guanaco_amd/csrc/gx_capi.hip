@@ -751,6 +751,9 @@ struct gx_volume_plan {
   // table (bp_coefficients), rebuilt by every backprojection; win: the staged
   // kernel's per-(tile, tilt) windows. Tilt tables hold bp_tilt_pad(T) rows.
   DevBuf coef, coef_r, win, wscale, tmax, ramp, mask, defocus, F, G, slab;
+  // krange[t]: the first and last defocus slab any voxel of the volume reads
+  // at tilt t (slab plans); the filter passes compute only those
+  DevBuf krange;
   // per-row rotation centres: row_off[t * Y + y], the tilt's offset for row
   // y (reduced storage: offset - 1/2); empty when one centre serves all rows
   DevBuf row_off;
@@ -852,7 +855,7 @@ struct EventPair {
 namespace {
 
 constexpr size_t kPlanAlign = 256;
-constexpr int kPlanBuffers = 13;
+constexpr int kPlanBuffers = 14;
 
 struct PlanSizes {
   int YB, NYB, XB, L, chunk, band;
@@ -937,7 +940,8 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
                         sizeof(int2) +
                     sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf +
                     (d->centres ? sizeof(float) * T * Y : 0) +
-                    (ps->ctf ? sizeof(float2) * bs_entries(X, Y) : 0);
+                    (ps->ctf ? sizeof(float2) * bs_entries(X, Y) : 0) +
+                    (ndf > 1 ? sizeof(int2) * T : 0);
   // every buffer starts on a kPlanAlign boundary inside a workspace
   ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
               ps->small_bytes + kPlanBuffers * kPlanAlign;
@@ -1041,7 +1045,8 @@ void plan_buffers(gx_volume_plan *p, const PlanSizes &ps,
          {&p->F, ps.f_bytes},
          {&p->G, ps.g_bytes},
          {&p->row_off, sizeof(float) * p->h_row_off.size()},
-         {&p->bs, sizeof(float2) * p->h_bs.size()}};
+         {&p->bs, sizeof(float2) * p->h_bs.size()},
+         {&p->krange, p->ndf > 1 ? sizeof(int2) * T : 0}};
 }
 
 // Uploads the tables and initialises the device state on stream s.
@@ -1061,6 +1066,12 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
                              p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
                              s));
   }
+  // the slabs each tilt's voxels read (the slab choice uses only the tilt's
+  // cos and sin terms, which both tables hold alike)
+  if (p->ndf > 1)
+    GX_HIP(launch_bp_slab_range(p->coef.as<float4>(), p->T, p->W, p->Z,
+                                p->dscale, p->doffset, p->ndf,
+                                p->krange.as<int2>(), s));
   GX_HIP(hipMemcpyAsync(p->wscale.p, p->h_wscale.data(), sizeof(float) * T,
                         hipMemcpyHostToDevice, s));
   if (!p->h_bs.empty())
@@ -1177,6 +1188,7 @@ extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
       bluestein_view(p->bs.as<float2>(), (size_t)p->X, (size_t)p->MX);
   const gx::BluesteinTable by = bluestein_view(
       p->bs.as<float2>() + p->bs_y, (size_t)p->Y, (size_t)p->MY);
+  const int2 *kr = p->ndf > 1 ? p->krange.as<int2>() : nullptr;
   if (p->ctf) {
     // The 2D correction needs whole projections: every row is transformed,
     // the column pass writes and the row pass reads only the band's rows.
@@ -1193,25 +1205,29 @@ extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
         GX_HIP(launch_rows_r2c(img, Y * X, p->X, p->Y, p->XB, tc,
                                p->F.as<float2>(), tmax, t0, s));
       }
+      // only the slabs the volume reads (per tilt of the launch for the
+      // column pass, per tilt of the plan for the row pass)
       if (p->cols_bs) {
         GX_HIP(launch_cols_bs(p->F.as<float2>(), p->mask.as<int8_t>(),
                               p->G.as<float2>(), p->Y, p->XB, p->ndf, tc,
-                              p->MY, by, s, by0, by0 + bny));
+                              p->MY, by, s, by0, by0 + bny,
+                              kr ? kr + t0 : nullptr));
       } else {
         GX_HIP(launch_cols(p->F.as<float2>(), p->mask.as<int8_t>(),
                            p->G.as<float2>(), p->Y, p->XB, p->ndf, tc, s,
-                           by0, by0 + bny));
+                           by0, by0 + bny, kr ? kr + t0 : nullptr));
       }
       if (p->rows_bs) {
         GX_HIP(launch_rows_c2r_filter_bs(
             p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
             p->ramp.as<float>(), p->wscale.as<float>(), tmax, p->nrm_mul,
-            p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s, by0, bny));
+            p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s, by0, bny,
+            kr));
       } else {
         GX_HIP(launch_rows_c2r_filter(
             p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
             p->wscale.as<float>(), tmax, p->nrm_mul,
-            p->storage, p->slab.p, p->NYB, p->YB, s, by0, bny));
+            p->storage, p->slab.p, p->NYB, p->YB, s, by0, bny, kr));
       }
     }
   } else {
@@ -1233,7 +1249,7 @@ extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
     GX_HIP(launch_filter_rows(src, st, sk, sy, p->X, p->L, bny, p->T, p->ndf,
                               p->ramp.as<float>(), p->wscale.as<float>(), tmax,
                               p->nrm_mul, p->storage, p->slab.p, p->NYB, p->YB,
-                              s));
+                              s, nullptr, kr));
   }
   p->slab_y0 = (int)y0;
   p->slab_y1 = (int)y1;
@@ -1364,6 +1380,22 @@ extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
   if (d_slab) *d_slab = p->slab.p;
   if (bytes) *bytes = p->slab_bytes;
   if (rows_per_block) *rows_per_block = p->YB;
+  return GX_OK;
+}
+
+extern "C" int gx_volume_plan_slab_range(gx_volume_plan *p, int *first,
+                                         int *last) {
+  GX_CHECK(p != nullptr && first != nullptr && last != nullptr);
+  std::vector<int2> r(p->T, make_int2(0, 0));
+  if (p->ndf > 1) {
+    GX_HIP(hipSetDevice(p->device));
+    GX_HIP(hipMemcpy(r.data(), p->krange.p, sizeof(int2) * p->T,
+                     hipMemcpyDeviceToHost));
+  }
+  for (int t = 0; t < p->T; ++t) {
+    first[t] = r[t].x;
+    last[t] = r[t].y;
+  }
   return GX_OK;
 }
 

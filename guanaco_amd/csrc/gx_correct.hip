@@ -315,7 +315,7 @@ __global__ __launch_bounds__(CW *(Y / default_e(Y)))
 __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols(
     const float2 *__restrict__ F, const M *__restrict__ mask,
     float2 *__restrict__ G, int XB, int ndf, const float2 *__restrict__ tw,
-    int ylo, int yhi) {
+    int ylo, int yhi, const int2 *__restrict__ krange) {
   constexpr int E = default_e(Y);
   constexpr int NT = Y / E;
   constexpr int H = CB / CW;  // workgroups per column block
@@ -351,7 +351,9 @@ __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols(
       type;
   constexpr bool kRun = sizeof(M) == 1 && sizeof(MaskRun) == E;
   const size_t run0 = ((size_t)cb * NT + tid) * CB + ci;  // in E-byte runs
-  for (int k = 0; k < ndf; ++k) {
+  // only the defoci the volume reads at this tilt (all without krange)
+  const int2 kr = krange != nullptr ? krange[t] : make_int2(0, ndf - 1);
+  for (int k = kr.x; k <= kr.y; ++k) {
     // the thread's row and column indices pass through an empty asm each
     // defocus, so the per-row store addresses are formed in the iteration
     // instead of being held across the loop
@@ -501,7 +503,8 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
                       int t0, int T, const float *__restrict__ wscale,
                       const uint32_t *__restrict__ tmax, float nrm_mul,
                       typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-                      int YB, const float2 *__restrict__ tw, int y0, int ny) {
+                      int YB, const float2 *__restrict__ tw, int y0, int ny,
+                      const int2 *__restrict__ krange) {
   constexpr int L = 2 * X;
   constexpr int EL = default_e(L);
   constexpr int EX = EL / 2;
@@ -512,6 +515,7 @@ __global__ __launch_bounds__(2 * (2 * X / default_e(2 * X))) void
   __shared__ float2 s[2][fft_lds_len(L, EL)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
+  if (krange != nullptr && (k < krange[t].x || k > krange[t].y)) return;
   // first row of the slot within the band [y0, y0 + ny) the slab holds
   const int r0 = blockIdx.x * (4 * IT) + p * (2 * IT);
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
@@ -750,7 +754,8 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_rows_c2r_ramp(
     const float2 *__restrict__ G, int Y, int XB, int ndf, int t0, int T,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-    int YB, const float2 *__restrict__ tw, int y0, int ny) {
+    int YB, const float2 *__restrict__ tw, int y0, int ny,
+    const int2 *__restrict__ krange) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
   constexpr bool WAVE = NT <= 64;
@@ -775,6 +780,8 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_rows_c2r_ramp(
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
   const int k = (int)((lb / nry) % (unsigned)ndf);
   const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
+  // a defocus the volume does not read at this tilt: nothing to compute
+  if (krange != nullptr && (k < krange[t].x || k > krange[t].y)) return;
   const int R0 = (int)(lb % nry) * RB;
   const int ya0 = y0 + R0;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
@@ -839,7 +846,7 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_filter_ramp(
     int Y, int T, int ndf, const float *__restrict__ ramp,
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
-    int YB, const float2 *__restrict__ tw) {
+    int YB, const float2 *__restrict__ tw, const int2 *__restrict__ krange) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
   constexpr bool WAVE = NT <= 64;
@@ -854,6 +861,7 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_filter_ramp(
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
   const int t = (int)((lb / nry) % (unsigned)T);
   const int k = (int)(lb / (nry * (unsigned)T));
+  if (krange != nullptr && (k < krange[t].x || k > krange[t].y)) return;
   const int R0 = (int)(lb % nry) * (2 * P);
   const int ya = R0 + 2 * p, yb = ya + 1;
   const float *base = src + t * st + k * sk;
@@ -1209,7 +1217,8 @@ template <int M, int CW>
 __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
     const float2 *__restrict__ F, const int8_t *__restrict__ mask,
     float2 *__restrict__ G, int Y, int XB, int ndf,
-    const float2 *__restrict__ tw, BluesteinTable by, int ylo, int yhi) {
+    const float2 *__restrict__ tw, BluesteinTable by, int ylo, int yhi,
+    const int2 *__restrict__ krange) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
   static_assert(CB % CW == 0, "column groups split a block");
@@ -1228,7 +1237,8 @@ __global__ __launch_bounds__(CW *(M / default_e(M))) void k_cols_bs(
     v[m] = y < Y ? src[(size_t)y * CB + ci] : make_float2(0.0f, 0.0f);
   }
   bluestein<M, E, -1, CW>(v, s + c, tid, twm, by);
-  for (int k = 0; k < ndf; ++k) {
+  const int2 kr = krange != nullptr ? krange[t] : make_int2(0, ndf - 1);
+  for (int k = kr.x; k <= kr.y; ++k) {
     // as in k_cols: indices laundered and twiddles reloaded per defocus, so
     // that addresses and twiddles are not held across the loop
     int tk = tid, ck = ci;
@@ -1298,12 +1308,14 @@ __global__ __launch_bounds__(P * (M / default_e(M))) void
                          const uint32_t *__restrict__ tmax, float nrm_mul,
                          typename StoreT<STORE>::T *__restrict__ slab,
                          int NYB, int YB, const float2 *__restrict__ tw,
-                         BluesteinTable bx, int y0, int ny) {
+                         BluesteinTable bx, int y0, int ny,
+                         const int2 *__restrict__ krange) {
   constexpr int E = default_e(M);
   constexpr int NT = M / E;
   __shared__ float2 s[P][fft_lds_len(M, E)];
   const int p = threadIdx.x / NT, tid = threadIdx.x % NT;
   const int k = blockIdx.y, tl = blockIdx.z, t = t0 + tl;
+  if (krange != nullptr && (k < krange[t].x || k > krange[t].y)) return;
   // band-relative rows ra, rb of the slab; spectrum rows y0 + ra, y0 + rb
   const int ra = blockIdx.x * (2 * P) + 2 * p, rb = ra + 1;
   const int ya = y0 + ra, yb = ya + 1;
@@ -1375,7 +1387,8 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
 template <typename M>
 static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
                                 int Y, int XB, int ndf, int ntilts,
-                                hipStream_t s, int ylo, int yhi) {
+                                hipStream_t s, int ylo, int yhi,
+                                const int2 *krange) {
   if (yhi < 0) yhi = Y;
   hipError_t err;
   const float2 *tw = twiddles(&err);
@@ -1386,7 +1399,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   case n:                                                                  \
     hipLaunchKernelGGL((k_cols<n, M, CB>), grid,                           \
                        dim3(CB * (n / default_e(n))), 0, s, F, mask, G, XB, \
-                       ndf, tw, ylo, yhi);                                 \
+                       ndf, tw, ylo, yhi, krange);                         \
     break;
     C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
     C_(2048) C_(4096)
@@ -1396,7 +1409,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
       // halves of a column block on one XCD
       hipLaunchKernelGGL((k_cols<8192, M, 2>), dim3(2 * XB * ntilts),
                          dim3(2 * (8192 / default_e(8192))), 0, s, F, mask, G,
-                         XB, ndf, tw, ylo, yhi);
+                         XB, ndf, tw, ylo, yhi, krange);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1406,14 +1419,16 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
 
 hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
                        int XB, int ndf, int ntilts, hipStream_t s, int ylo,
-                       int yhi) {
-  return launch_cols_t<int8_t>(F, mask, G, Y, XB, ndf, ntilts, s, ylo, yhi);
+                       int yhi, const int2 *krange) {
+  return launch_cols_t<int8_t>(F, mask, G, Y, XB, ndf, ntilts, s, ylo, yhi,
+                               krange);
 }
 
 hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
                                 float2 *G, int Y, int XB, int ntilts,
                                 hipStream_t s) {
-  return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s, 0, Y);
+  return launch_cols_t<float>(F, transfer, G, Y, XB, 1, ntilts, s, 0, Y,
+                              nullptr);
 }
 
 // Rows per workgroup: the slab's row blocking YB, at most 16 (float32) or
@@ -1432,7 +1447,7 @@ static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
                                        const uint32_t *tmax, float nrm_mul,
                                        void *slab, int NYB, int YB,
                                        hipStream_t s, const float2 *tw,
-                                       int y0, int ny) {
+                                       int y0, int ny, const int2 *krange) {
   using ST = typename StoreT<STORE>::T;
   dim3 grid((ny + 4 * IT - 1) / (4 * IT), ndf, ntilts);
   switch (X) {
@@ -1441,7 +1456,7 @@ static hipError_t launch_c2r_filter_it(const float2 *G, int X, int Y, int XB,
     hipLaunchKernelGGL((k_rows_c2r_filter<n, STORE, IT>), grid,                \
                        dim3(2 * (2 * n / default_e(2 * n))), 0, s, G, Y, XB,   \
                        ndf, t0, T, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
-                       tw, y0, ny);                                            \
+                       tw, y0, ny, krange);                                    \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128)
 #undef C_
@@ -1466,11 +1481,13 @@ static void launch_c2r_ramp_x(dim3 grid, const float2 *G, int Y, int XB,
                               int ndf, int t0, int T, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, void *slab,
                               int NYB, int YB, hipStream_t s,
-                              const float2 *tw, int y0, int ny) {
+                              const float2 *tw, int y0, int ny,
+                              const int2 *krange) {
   using ST = typename StoreT<STORE>::T;
   hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P, IT>), grid,
                      dim3(P * (X / default_e(X))), 0, s, G, Y, XB, ndf, t0, T,
-                     wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw, y0, ny);
+                     wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw, y0, ny,
+                     krange);
 }
 
 template <int STORE, int P>
@@ -1479,13 +1496,13 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     const float *wscale, const uint32_t *tmax,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw, int y0,
-                                    int ny) {
+                                    int ny, const int2 *krange) {
   // two row pairs per slot at 4096 and 8192 points with 16-bit storage
   const int rb = X >= 4096 && STORE != kF32 ? 4 * P : 2 * P;
   const size_t nb = (size_t)((ny + rb - 1) / rb) * ndf * ntilts;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
-#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny
+#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, krange
   switch (X) {
     case 256: launch_c2r_ramp_x<256, STORE, P>(A_); break;
     case 512: launch_c2r_ramp_x<512, STORE, P>(A_); break;
@@ -1528,8 +1545,8 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
                                       hipStream_t s, const float2 *tw, int y0,
-                                      int ny) {
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny
+                                      int ny, const int2 *krange) {
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, krange
   if (X >= 256 && YB >= 8) {
     switch (c2r_ramp_slots(X, YB)) {
       case 1: return launch_c2r_ramp_p<STORE, 1>(A_);
@@ -1557,7 +1574,8 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
-                                  hipStream_t s, int y0, int ny) {
+                                  hipStream_t s, int y0, int ny,
+                                  const int2 *krange) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
@@ -1568,15 +1586,15 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
     case kF32:
       return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw, y0, ny);
+                                       s, tw, y0, ny, krange);
     case kF16:
       return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw, y0, ny);
+                                       s, tw, y0, ny, krange);
     case kBF16:
       return launch_c2r_filter_t<kBF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                         wscale, tmax, nrm_mul, slab,
-                                        NYB, YB, s, tw, y0, ny);
+                                        NYB, YB, s, tw, y0, ny, krange);
   }
   return hipErrorInvalidValue;
 }
@@ -1661,12 +1679,13 @@ static void launch_filter_ramp_x(dim3 grid, const float *src, size_t st,
                                  int ndf, const float *ramp,
                                  const float *wscale, const uint32_t *tmax,
                                  float nrm_mul, void *slab, int NYB, int YB,
-                                 hipStream_t s, const float2 *tw) {
+                                 hipStream_t s, const float2 *tw,
+                                 const int2 *krange) {
   using ST = typename StoreT<STORE>::T;
   hipLaunchKernelGGL((k_filter_ramp<X, STORE, P>), grid,
                      dim3(P * (X / default_e(X))), 0, s, src, st, sk, sy, N,
                      Y, T, ndf, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB,
-                     YB, tw);
+                     YB, tw, krange);
 }
 
 template <int STORE, int P>
@@ -1676,11 +1695,12 @@ static hipError_t launch_filter_ramp_p(const float *src, size_t st, size_t sk,
                                        const float *wscale,
                                        const uint32_t *tmax, float nrm_mul,
                                        void *slab, int NYB, int YB,
-                                       hipStream_t s, const float2 *tw) {
+                                       hipStream_t s, const float2 *tw,
+                                       const int2 *krange) {
   const size_t nb = (size_t)((Y + 2 * P - 1) / (2 * P)) * T * ndf;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
-#define A_ grid, src, st, sk, sy, N, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+#define A_ grid, src, st, sk, sy, N, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, krange
   switch (L / 2) {
     case 256: launch_filter_ramp_x<256, STORE, P>(A_); break;
     case 512: launch_filter_ramp_x<512, STORE, P>(A_); break;
@@ -1717,8 +1737,9 @@ static hipError_t launch_filter_ramp_t(const float *src, size_t st, size_t sk,
                                        const float *wscale,
                                        const uint32_t *tmax, float nrm_mul,
                                        void *slab, int NYB, int YB,
-                                       hipStream_t s, const float2 *tw) {
-#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+                                       hipStream_t s, const float2 *tw,
+                                       const int2 *krange) {
+#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, krange
   switch (c2r_ramp_slots(L / 2, YB)) {
     case 1: return launch_filter_ramp_p<STORE, 1>(A_);
     case 2: return launch_filter_ramp_p<STORE, 2>(A_);
@@ -1733,7 +1754,7 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
                               const float *ramp, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, int store,
                               void *slab, int NYB, int YB, hipStream_t s,
-                              const float2 *tw) {
+                              const float2 *tw, const int2 *krange) {
   if (tw == nullptr) {
     hipError_t err;
     tw = twiddles(&err);
@@ -1742,7 +1763,7 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
   // 512- to 16384-point filters: four half-length transforms per row pair
   // (16384 points, rows of 4097 to 8192: at any row blocking, one slot)
   if (L >= 512 && (L == 16384 || (L <= 8192 && YB >= 8))) {
-#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw
+#define A_ src, st, sk, sy, N, L, Y, T, ndf, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, krange
     switch (store) {
       case kF32: return launch_filter_ramp_t<kF32>(A_);
       case kF16: return launch_filter_ramp_t<kF16>(A_);
@@ -1830,7 +1851,7 @@ hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
 hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                           int Y, int XB, int ndf, int ntilts, int M,
                           const BluesteinTable &by, hipStream_t s, int ylo,
-                          int yhi) {
+                          int yhi, const int2 *krange) {
   if (yhi < 0 || yhi > Y) yhi = Y;
   hipError_t err;
   const float2 *tw = twiddles(&err);
@@ -1843,7 +1864,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
   case n:                                                                     \
     hipLaunchKernelGGL((k_cols_bs<n, CB>), dim3(XB, ntilts),                  \
                        dim3(CB * (n / default_e(n))), 0, s, F, mask, G, Y,    \
-                       XB, ndf, tw, by, ylo, yhi);                            \
+                       XB, ndf, tw, by, ylo, yhi, krange);                    \
     break;
     C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024) C_(2048)
 #undef C_
@@ -1851,7 +1872,7 @@ hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
   case n:                                                                     \
     hipLaunchKernelGGL((k_cols_bs<n, 1>), dim3(XB * CB, ntilts),              \
                        dim3(n / default_e(n)), 0, s, F, mask, G, Y, XB, ndf,  \
-                       tw, by, ylo, yhi);                                     \
+                       tw, by, ylo, yhi, krange);                             \
     break;
     C_(4096) C_(8192) C_(16384)
 #undef C_
@@ -1893,7 +1914,7 @@ static hipError_t launch_c2r_filter_bs_t(const float2 *G, int X, int Y,
                                          void *slab, int NYB, int YB, int M,
                                          const BluesteinTable &bx,
                                          hipStream_t s, const float2 *tw,
-                                         int y0, int ny) {
+                                         int y0, int ny, const int2 *krange) {
   using ST = typename StoreT<STORE>::T;
   switch (M) {
 #define C_(n)                                                                  \
@@ -1903,7 +1924,7 @@ static hipError_t launch_c2r_filter_bs_t(const float2 *G, int X, int Y,
     hipLaunchKernelGGL((k_rows_c2r_filter_bs<n, STORE>), grid,                 \
                        dim3(P * (n / default_e(n))), 0, s, G, X, Y, XB, ndf,   \
                        t0, T, ramp, wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, \
-                       tw, bx, y0, ny);                                        \
+                       tw, bx, y0, ny, krange);                                \
     break;                                                                     \
   }
     GX_BS_CASES(C_)
@@ -1920,14 +1941,15 @@ hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
                                      const uint32_t *tmax, float nrm_mul,
                                      int store, void *slab, int NYB, int YB,
                                      int M, const BluesteinTable &bx,
-                                     hipStream_t s, int y0, int ny) {
+                                     hipStream_t s, int y0, int ny,
+                                     const int2 *krange) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
   if (ny < 0) ny = Y - y0;
   if (y0 < 0 || y0 % YB != 0 || ny <= 0 || y0 + ny > Y)
     return hipErrorInvalidValue;
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, M, bx, s, tw, y0, ny
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, ramp, wscale, tmax, nrm_mul, slab, NYB, YB, M, bx, s, tw, y0, ny, krange
   switch (store) {
     case kF32:
       return launch_c2r_filter_bs_t<kF32>(A_);

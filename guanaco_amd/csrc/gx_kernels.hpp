@@ -64,10 +64,13 @@ hipError_t launch_rows_r2c_bs(const float *img, size_t tstride, int X, int Y,
                               hipStream_t s);
 // ylo, yhi: the rows of G written (the band the row pass reads; yhi < 0:
 // every row).
+// krange (optional): per tilt of the launch, the first and last defocus to
+// compute; the others are skipped.
 hipError_t launch_cols_bs(const float2 *F, const int8_t *mask, float2 *G,
                           int Y, int XB, int ndf, int ntilts, int M,
                           const BluesteinTable &by, hipStream_t s,
-                          int ylo = 0, int yhi = -1);
+                          int ylo = 0, int yhi = -1,
+                          const int2 *krange = nullptr);
 hipError_t launch_rows_c2r_bs(const float2 *G, int X, int Y, int XB, int ndf,
                               int ntilts, float *out, int M,
                               const BluesteinTable &bx, hipStream_t s);
@@ -77,7 +80,8 @@ hipError_t launch_rows_c2r_filter_bs(const float2 *G, int X, int Y, int XB,
                                      const uint32_t *tmax, float nrm_mul,
                                      int store, void *slab, int NYB, int YB,
                                      int M, const BluesteinTable &bx,
-                                     hipStream_t s, int y0 = 0, int ny = -1);
+                                     hipStream_t s, int y0 = 0, int ny = -1,
+                                     const int2 *krange = nullptr);
 
 // CTF parameters in the reference's units (ctf.h:167-178): A and rad
 struct CtfParams {
@@ -102,7 +106,8 @@ hipError_t launch_rows_r2c(const float *img, size_t tstride, int X, int Y,
                            int t0, hipStream_t s);
 hipError_t launch_cols(const float2 *F, const int8_t *mask, float2 *G, int Y,
                        int XB, int ndf, int ntilts, hipStream_t s,
-                       int ylo = 0, int yhi = -1);
+                       int ylo = 0, int yhi = -1,
+                       const int2 *krange = nullptr);
 // Same column pass with a real transfer function (one "defocus").
 hipError_t launch_cols_transfer(const float2 *F, const float *transfer,
                                 float2 *G, int Y, int XB, int ntilts,
@@ -112,7 +117,8 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const float *wscale,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
-                                  hipStream_t s, int y0 = 0, int ny = -1);
+                                  hipStream_t s, int y0 = 0, int ny = -1,
+                                  const int2 *krange = nullptr);
 hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
                            int ntilts, float *out, hipStream_t s);
 // tw: a twiddle table (twiddle_host_table() values) to use instead of the
@@ -122,7 +128,8 @@ hipError_t launch_filter_rows(const float *src, size_t st, size_t sk,
                               const float *ramp, const float *wscale,
                               const uint32_t *tmax, float nrm_mul, int store,
                               void *slab, int NYB, int YB, hipStream_t s,
-                              const float2 *tw = nullptr);
+                              const float2 *tw = nullptr,
+                              const int2 *krange = nullptr);
 hipError_t launch_absmax_rows(const float *src, size_t st, size_t sk,
                               size_t sy, int N, int Y, int T, int ndf,
                               uint32_t *tmax, hipStream_t s);
@@ -140,6 +147,11 @@ size_t bp_window_entries(int W, int Z, int T);
 hipError_t launch_bp_windows(const float4 *coef, int exact, int T, int W,
                              int Z, float dscale, float doffset, int ndf,
                              int2 *win, hipStream_t s);
+// range[t] = (first, last) slab that any voxel of the W x Z grid reads at
+// tilt t (the filter passes compute only those).
+hipError_t launch_bp_slab_range(const float4 *coef, int T, int W, int Z,
+                                float dscale, float doffset, int ndf,
+                                int2 *range, hipStream_t s);
 
 // Seeded synthetic tilt series (ellipsoid phantom, optional CTF, noise).
 struct SynthArgs {

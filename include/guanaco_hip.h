@@ -255,9 +255,15 @@ int gx_volume_rows_per_block(gx_volume_plan *plan, size_t *rows);
 int gx_volume_plan_set_backprojector(gx_volume_plan *plan, int kind);
 int gx_volume_plan_get_backprojector(gx_volume_plan *plan, int *kind,
                                      int *slabs_per_tilt);
-/* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage. */
+/* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage.
+ * Plans with several defocus slabs compute, at each tilt t, only the slabs
+ * [first[t], last[t]] that some voxel of the volume reads (no reference
+ * counterpart: the reference corrects every slab); the others hold no data. */
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);
+/* first[t], last[t] for every tilt (host arrays of num_angles ints; 0 and 0
+ * with one slab). Synchronous. */
+int gx_volume_plan_slab_range(gx_volume_plan *plan, int *first, int *last);
 
 /* ---- autotuning (no reference counterpart) --------------------------------
  * The backprojection tile of a plan shape is chosen by measurement. A tile

@@ -296,6 +296,42 @@ def test_volume_plain_long_rows_match_oracle(capi, case):
         assert rel_l2(got, expect) < bound, storage
 
 
+def test_slab_range_is_every_slab_the_voxels_read(capi):
+    # The filter computes, per tilt, only the defocus slabs some voxel reads:
+    # the plan's range must be the min and max of the reference's slab
+    # choice (reconstructor.cpp:66-73, float arithmetic) over every voxel,
+    # here by brute force; and the volume is the oracle chain's, which
+    # computes every slab.
+    X, Y, T, ndf, Z, ps, df = 256, 4, 61, 16, 64, 1.0, 25000.0
+    proj, angles, centre, dfs, rel = ctf_problem(X, Y, T, ndf, ps, df, 11)
+    l = O.electron_wavelength(300000)
+    plan = C.VolumePlan(X, Y, angles, Z, num_defocus=ndf, pixel_size=ps,
+                        centre=X / 2.0, min_defocus=rel[0],
+                        max_defocus=rel[1], storage="f16", defocus=dfs,
+                        wavelength=l, Cs=2.7e7)
+    first, last = plan.slab_range()
+    plan.destroy()
+    tc = O.tilt_coefficients(angles, X / 2.0)
+    f = numpy.float32
+    dscale = f(f(ndf) * f(ps) / (f(rel[1]) - f(rel[0])))
+    doffset = f(-dscale * f(f(rel[0]) / f(ps)))
+    x = (numpy.arange(X) - 0.5 * X + 0.5).astype(f)[None, :]
+    z = (numpy.arange(Z) - 0.5 * Z + 0.5).astype(f)[:, None]
+    for t in range(T):
+        c, s_ = tc[t, 0], tc[t, 1]
+        h = (-x * s_) - (z * c)
+        d = h * dscale + doffset
+        k = numpy.clip(numpy.floor(d.astype(numpy.float64) + 0.5), 0,
+                       ndf - 1).astype(int)
+        assert (first[t], last[t]) == (k.min(), k.max()), t
+    assert (last - first + 1).sum() < T * ndf  # some slabs are skipped
+    corrected, mn, mx = O.correct_projections(
+        proj, centre, ps, 300, df, ndf, None, 2.7, 0, 0, 0)
+    expect = O.reconstruct(corrected, angles, X / 2.0, ps, mn, mx, depth=Z)
+    got = gpu_volume(proj, angles, X, Y, Z, ndf, ps, rel, "f32", dfs)
+    assert rel_l2(got, expect) < 1e-5
+
+
 # ------------------------------------------- graphs and synthetic data ----
 
 def test_graph_replays_match_eager(capi):
