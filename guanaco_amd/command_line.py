@@ -130,7 +130,8 @@ def reconstruct_parser():
         "--storage", dest="storage", default="f32",
         choices=["f32", "f16", "bf16"],
         help="Precision of the filtered projections on the GPU "
-        "(accumulation is float32)")
+        "(accumulation is float32; f16 stays within 1e-3 rel-L2 of FP64, "
+        "bf16 does not)")
     parser.add_argument(
         "--depth", dest="depth", default=None, type=int,
         help="Reconstruct only the central DEPTH rows of each slice")

@@ -296,11 +296,13 @@ __device__ __forceinline__ float mask_value(M m) {
 // CW columns per workgroup (WPE waves per SIMD bound the registers). With
 // CW < CB the grid is 1D over (column part, column block, tilt), remapped so
 // that each XCD takes consecutive ones: the parts of a column block, which
-// share its 32-byte row segments, meet in one L2. The launches use CW = CB:
-// a wave then stores 16 rows x 4 columns, whole 128-byte lines of G; with
-// one column per workgroup at 4096 points the partial-line stores doubled
-// the bytes written to HBM (4.5 GB per tilt at 32 defoci, 2.15 GB
-// compulsory).
+// share its 32-byte row segments, meet in one L2. The launches use CW = CB
+// up to 2048 points: a wave then stores 16 rows x 4 columns, whole 128-byte
+// lines of G. With one column per workgroup at 4096 points the partial-line
+// stores doubled the bytes written to HBM (4.5 GB per tilt at 32 defoci,
+// 2.15 GB compulsory); at 4096 and 8192 points two columns per workgroup
+// (two 512-thread workgroups per CU at 4096, so one's barriers overlap the
+// other's work) measured faster than four (C4 column pass 58.3 -> 52.9 ms).
 //
 // Registers: the forward spectrum (2E) stays live across the defocus loop;
 // the per-row store addresses and the twiddles (about 50 registers) would
@@ -1402,15 +1404,18 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
                        ndf, tw, ylo, yhi, krange);                         \
     break;
     C_(4) C_(8) C_(16) C_(32) C_(64) C_(128) C_(256) C_(512) C_(1024)
-    C_(2048) C_(4096)
+    C_(2048)
 #undef C_
-    case 8192:
-      // two columns per 1024-thread workgroup (139 KB of LDS), the two
-      // halves of a column block on one XCD
-      hipLaunchKernelGGL((k_cols<8192, M, 2>), dim3(2 * XB * ntilts),
-                         dim3(2 * (8192 / default_e(8192))), 0, s, F, mask, G,
-                         XB, ndf, tw, ylo, yhi, krange);
-      break;
+    // two columns per workgroup (512 threads and 70 KB of LDS at 4096, 1024
+    // and 139 KB at 8192), the two halves of a column block on one XCD
+#define C_(n)                                                              \
+  case n:                                                                  \
+    hipLaunchKernelGGL((k_cols<n, M, 2>), dim3(2 * XB * ntilts),           \
+                       dim3(2 * (n / default_e(n))), 0, s, F, mask, G, XB, \
+                       ndf, tw, ylo, yhi, krange);                         \
+    break;
+    C_(4096) C_(8192)
+#undef C_
     default:
       return hipErrorInvalidValue;
   }

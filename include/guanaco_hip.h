@@ -31,8 +31,8 @@ extern "C" {
 
 /* Storage precision of filtered projections (accumulation is always FP32). */
 #define GX_STORE_F32 0  /* exact reference FP32 arithmetic */
-#define GX_STORE_F16 1
-#define GX_STORE_BF16 2
+#define GX_STORE_F16 1  /* within 1e-3 rel-L2 of the FP64 reconstruction */
+#define GX_STORE_BF16 2 /* 8-bit mantissa: about 2e-3, above that gate */
 
 /* ---- status / devices ---------------------------------------------------- */
 
