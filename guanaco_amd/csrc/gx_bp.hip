@@ -280,36 +280,6 @@ __global__ void k_bp_windows(const float4 *__restrict__ coef, int T, int Tpad,
   win[i] = r;
 }
 
-// Slabs the volume reads at each tilt: slab_of at the four corners of the
-// W x Z voxel grid (monotone in x and in z, so they bound every voxel),
-// with the backprojectors' own arithmetic: range[t] = (first, last).
-__global__ void k_bp_slab_range(const float4 *__restrict__ coef, int T, int W,
-                                int Z, float dscale, float doffset, int ndf,
-                                int2 *__restrict__ range) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= T) return;
-  const float4 cf = coef[t];
-  const float x0 = voxel_coord(0, W), x1 = voxel_coord(W - 1, W);
-  const float z0 = voxel_coord(0, Z), z1 = voxel_coord(Z - 1, Z);
-  int lo = ndf - 1, hi = 0;
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int k = slab_of((c & 1) ? x1 : x0, (c & 2) ? z1 : z0, cf, dscale,
-                          doffset, ndf);
-    lo = min(lo, k);
-    hi = max(hi, k);
-  }
-  range[t] = make_int2(lo, hi);
-}
-
-hipError_t launch_bp_slab_range(const float4 *coef, int T, int W, int Z,
-                                float dscale, float doffset, int ndf,
-                                int2 *range, hipStream_t s) {
-  hipLaunchKernelGGL(k_bp_slab_range, dim3((T + 63) / 64), dim3(64), 0, s,
-                     coef, T, W, Z, dscale, doffset, ndf, range);
-  return hipGetLastError();
-}
-
 size_t bp_window_entries(int W, int Z, int T) {
   const size_t nbx = (W + kTX - 1) / kTX, nbz = (Z + kTZ - 1) / kTZ;
   return nbx * nbz * (size_t)bp_tilt_pad(T);

@@ -123,6 +123,36 @@ void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
   *doffset = -ds * (mn / ps);
 }
 
+// The first and last defocus slab any voxel of the W x Z grid reads at each
+// tilt: the backprojectors' slab choice (gx_bp.hip: slab_of, the voxel
+// coordinates of voxel_coord) evaluated here in the same float operations at
+// the four corners of the grid; the choice is monotone in x and in z, so the
+// corners bound every voxel. coef: the per-tilt table of tilt_geometry.
+void slab_ranges(size_t T, const float *coef, int W, int Z, float dscale,
+                 float doffset, int ndf, std::vector<int2> &out) {
+  auto coord = [](int i, int n) {
+    return (float)((double)i - 0.5 * (double)n + 0.5);
+  };
+  const float xs[2] = {coord(0, W), coord(W - 1, W)};
+  const float zs[2] = {coord(0, Z), coord(Z - 1, Z)};
+  out.assign(T, make_int2(0, 0));
+  for (size_t t = 0; t < T; ++t) {
+    const float c = coef[4 * t + 0], s = coef[4 * t + 1];
+    int lo = ndf - 1, hi = 0;
+    for (int q = 0; q < 4; ++q) {
+      const float x = xs[q & 1], z = zs[q >> 1];
+      const float h = -x * s - z * c;
+      const float d = h * dscale + doffset;
+      float n = std::nearbyint(d);
+      if (d - n == 0.5f) n += 1.0f;
+      const int k = std::min(std::max((int)n, 0), ndf - 1);
+      lo = std::min(lo, k);
+      hi = std::max(hi, k);
+    }
+    out[t] = make_int2(lo, hi);
+  }
+}
+
 // Slabs the LDS-staged backprojector must hold per tilt: a 32 x 8 voxel tile
 // spans at most 31.79 units of h (31 |s| + 7 |c|), i.e. 31.79 |dscale| slab
 // widths, so at most ceil of that plus one consecutive slabs (31.9 leaves a
@@ -752,8 +782,12 @@ struct gx_volume_plan {
   // kernel's per-(tile, tilt) windows. Tilt tables hold bp_tilt_pad(T) rows.
   DevBuf coef, coef_r, win, wscale, tmax, ramp, mask, defocus, F, G, slab;
   // krange[t]: the first and last defocus slab any voxel of the volume reads
-  // at tilt t (slab plans); the filter passes compute only those
-  DevBuf krange;
+  // at tilt t (slab plans); the filter passes compute only those. pairs:
+  // every (tilt, defocus) they compute, tilt-major, pair_off[t] the first of
+  // tilt t (pair_off[T] = their number).
+  DevBuf krange, pairs;
+  std::vector<int2> h_krange, h_pairs;
+  std::vector<int> pair_off;
   // per-row rotation centres: row_off[t * Y + y], the tilt's offset for row
   // y (reduced storage: offset - 1/2); empty when one centre serves all rows
   DevBuf row_off;
@@ -855,7 +889,7 @@ struct EventPair {
 namespace {
 
 constexpr size_t kPlanAlign = 256;
-constexpr int kPlanBuffers = 14;
+constexpr int kPlanBuffers = 15;
 
 struct PlanSizes {
   int YB, NYB, XB, L, chunk, band;
@@ -941,7 +975,7 @@ int plan_sizes(const gx_volume_desc *d, PlanSizes *ps) {
                     sizeof(float) * (ps->L / 2 + 1) + sizeof(double) * ndf +
                     (d->centres ? sizeof(float) * T * Y : 0) +
                     (ps->ctf ? sizeof(float2) * bs_entries(X, Y) : 0) +
-                    (ndf > 1 ? sizeof(int2) * T : 0);
+                    (ndf > 1 ? sizeof(int2) * T * (1 + ndf) : 0);
   // every buffer starts on a kPlanAlign boundary inside a workspace
   ps->total = ps->slab_bytes + ps->mask_bytes + ps->f_bytes + ps->g_bytes +
               ps->small_bytes + kPlanBuffers * kPlanAlign;
@@ -1007,6 +1041,18 @@ void plan_setup(gx_volume_plan *p, const gx_volume_desc *d,
   }
   p->h_coef.assign(4 * (size_t)bp_tilt_pad((int)T), 0.0f);
   tilt_geometry(T, d->angles, d->centre, p->h_coef.data());
+  if (p->ndf > 1) {
+    slab_ranges(T, p->h_coef.data(), p->W, p->Z, p->dscale, p->doffset,
+                p->ndf, p->h_krange);
+    p->h_pairs.clear();
+    p->pair_off.assign(T + 1, 0);
+    for (size_t t = 0; t < T; ++t) {
+      p->pair_off[t] = (int)p->h_pairs.size();
+      for (int k = p->h_krange[t].x; k <= p->h_krange[t].y; ++k)
+        p->h_pairs.push_back(make_int2((int)t, k));
+    }
+    p->pair_off[T] = (int)p->h_pairs.size();
+  }
   p->h_wscale.assign(T, 1.0f);
   if (d->weights)
     for (size_t t = 0; t < T; ++t) p->h_wscale[t] = d->weights[t];
@@ -1046,7 +1092,8 @@ void plan_buffers(gx_volume_plan *p, const PlanSizes &ps,
          {&p->G, ps.g_bytes},
          {&p->row_off, sizeof(float) * p->h_row_off.size()},
          {&p->bs, sizeof(float2) * p->h_bs.size()},
-         {&p->krange, p->ndf > 1 ? sizeof(int2) * T : 0}};
+         {&p->krange, p->ndf > 1 ? sizeof(int2) * T : 0},
+         {&p->pairs, p->ndf > 1 ? sizeof(int2) * T * p->ndf : 0}};
 }
 
 // Uploads the tables and initialises the device state on stream s.
@@ -1066,12 +1113,15 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
                              p->dscale, p->doffset, p->ndf, p->win.as<int2>(),
                              s));
   }
-  // the slabs each tilt's voxels read (the slab choice uses only the tilt's
-  // cos and sin terms, which both tables hold alike)
-  if (p->ndf > 1)
-    GX_HIP(launch_bp_slab_range(p->coef.as<float4>(), p->T, p->W, p->Z,
-                                p->dscale, p->doffset, p->ndf,
-                                p->krange.as<int2>(), s));
+  // the slabs each tilt's voxels read, and the (tilt, defocus) list
+  if (p->ndf > 1) {
+    GX_HIP(hipMemcpyAsync(p->krange.p, p->h_krange.data(),
+                          sizeof(int2) * p->h_krange.size(),
+                          hipMemcpyHostToDevice, s));
+    GX_HIP(hipMemcpyAsync(p->pairs.p, p->h_pairs.data(),
+                          sizeof(int2) * p->h_pairs.size(),
+                          hipMemcpyHostToDevice, s));
+  }
   GX_HIP(hipMemcpyAsync(p->wscale.p, p->h_wscale.data(), sizeof(float) * T,
                         hipMemcpyHostToDevice, s));
   if (!p->h_bs.empty())
@@ -1224,10 +1274,12 @@ extern "C" int gx_volume_filter_rows(gx_volume_plan *p, const float *d_in,
             p->storage, p->slab.p, p->NYB, p->YB, p->MX, bx, s, by0, bny,
             kr));
       } else {
+        const int2 *pl = kr ? p->pairs.as<int2>() + p->pair_off[t0] : nullptr;
+        const int np = kr ? p->pair_off[t0 + tc] - p->pair_off[t0] : 0;
         GX_HIP(launch_rows_c2r_filter(
             p->G.as<float2>(), p->X, p->Y, p->XB, p->ndf, t0, tc, p->T,
             p->wscale.as<float>(), tmax, p->nrm_mul,
-            p->storage, p->slab.p, p->NYB, p->YB, s, by0, bny, kr));
+            p->storage, p->slab.p, p->NYB, p->YB, s, by0, bny, kr, pl, np));
       }
     }
   } else {
@@ -1386,15 +1438,9 @@ extern "C" int gx_volume_slabs(gx_volume_plan *p, void **d_slab, size_t *bytes,
 extern "C" int gx_volume_plan_slab_range(gx_volume_plan *p, int *first,
                                          int *last) {
   GX_CHECK(p != nullptr && first != nullptr && last != nullptr);
-  std::vector<int2> r(p->T, make_int2(0, 0));
-  if (p->ndf > 1) {
-    GX_HIP(hipSetDevice(p->device));
-    GX_HIP(hipMemcpy(r.data(), p->krange.p, sizeof(int2) * p->T,
-                     hipMemcpyDeviceToHost));
-  }
   for (int t = 0; t < p->T; ++t) {
-    first[t] = r[t].x;
-    last[t] = r[t].y;
+    first[t] = p->ndf > 1 ? p->h_krange[t].x : 0;
+    last[t] = p->ndf > 1 ? p->h_krange[t].y : 0;
   }
   return GX_OK;
 }

@@ -740,6 +740,9 @@ __device__ __forceinline__ void ramp_pair(
   }
 }
 
+// The grid is 1D over (row group, defocus, tilt), or over (row group, entry
+// of `pairs`) when the plan lists the (tilt, defocus) pairs its volume reads
+// (absolute tilts, those of this launch), so that no workgroup is idle.
 // The workgroup's P slots each take one row pair (IT = 1) and meet once, to
 // stage their outputs in LDS column by column (store_row_group), so that the
 // workgroup writes each column's RB = 2P rows (RB dividing the slab's row
@@ -757,7 +760,7 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_rows_c2r_ramp(
     const float *__restrict__ wscale, const uint32_t *__restrict__ tmax,
     float nrm_mul, typename StoreT<STORE>::T *__restrict__ slab, int NYB,
     int YB, const float2 *__restrict__ tw, int y0, int ny,
-    const int2 *__restrict__ krange) {
+    const int2 *__restrict__ pairs) {
   constexpr int E = default_e(X);
   constexpr int NT = X / E;
   constexpr bool WAVE = NT <= 64;
@@ -780,10 +783,17 @@ __attribute__((amdgpu_waves_per_eu(X >= 8192 ? 2 : 3))) void k_rows_c2r_ramp(
   // y0, and the spectrum rows read are y0 + R0 onwards.
   const unsigned nry = (unsigned)(ny + RB - 1) / RB;
   const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int k = (int)((lb / nry) % (unsigned)ndf);
-  const int tl = (int)(lb / (nry * (unsigned)ndf)), t = t0 + tl;
-  // a defocus the volume does not read at this tilt: nothing to compute
-  if (krange != nullptr && (k < krange[t].x || k > krange[t].y)) return;
+  // (tilt, defocus): from the list of those the volume reads, or every one
+  int k, tl;
+  if (pairs != nullptr) {
+    const int2 pk = pairs[lb / nry];
+    tl = pk.x - t0;
+    k = pk.y;
+  } else {
+    k = (int)((lb / nry) % (unsigned)ndf);
+    tl = (int)(lb / (nry * (unsigned)ndf));
+  }
+  const int t = t0 + tl;
   const int R0 = (int)(lb % nry) * RB;
   const int ya0 = y0 + R0;
   const float2 *g = G + ((size_t)tl * ndf + k) * XB * Y * CB;
@@ -1487,12 +1497,12 @@ static void launch_c2r_ramp_x(dim3 grid, const float2 *G, int Y, int XB,
                               const uint32_t *tmax, float nrm_mul, void *slab,
                               int NYB, int YB, hipStream_t s,
                               const float2 *tw, int y0, int ny,
-                              const int2 *krange) {
+                              const int2 *pairs) {
   using ST = typename StoreT<STORE>::T;
   hipLaunchKernelGGL((k_rows_c2r_ramp<X, STORE, P, IT>), grid,
                      dim3(P * (X / default_e(X))), 0, s, G, Y, XB, ndf, t0, T,
                      wscale, tmax, nrm_mul, (ST *)slab, NYB, YB, tw, y0, ny,
-                     krange);
+                     pairs);
 }
 
 template <int STORE, int P>
@@ -1501,13 +1511,15 @@ static hipError_t launch_c2r_ramp_p(const float2 *G, int X, int Y, int XB,
                                     const float *wscale, const uint32_t *tmax,
                                     float nrm_mul, void *slab, int NYB, int YB,
                                     hipStream_t s, const float2 *tw, int y0,
-                                    int ny, const int2 *krange) {
+                                    int ny, const int2 *pairs, int npairs) {
   // two row pairs per slot at 4096 and 8192 points with 16-bit storage
   const int rb = X >= 4096 && STORE != kF32 ? 4 * P : 2 * P;
-  const size_t nb = (size_t)((ny + rb - 1) / rb) * ndf * ntilts;
+  const size_t nb = (size_t)((ny + rb - 1) / rb) *
+                    (pairs != nullptr ? (size_t)npairs : (size_t)ndf * ntilts);
+  if (nb == 0) return hipSuccess;
   if (nb > 0xffffffffu) return hipErrorInvalidValue;
   dim3 grid((unsigned)nb);
-#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, krange
+#define A_ grid, G, Y, XB, ndf, t0, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, pairs
   switch (X) {
     case 256: launch_c2r_ramp_x<256, STORE, P>(A_); break;
     case 512: launch_c2r_ramp_x<512, STORE, P>(A_); break;
@@ -1550,8 +1562,9 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
                                       const uint32_t *tmax, float nrm_mul,
                                       void *slab, int NYB, int YB,
                                       hipStream_t s, const float2 *tw, int y0,
-                                      int ny, const int2 *krange) {
-#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, krange
+                                      int ny, const int2 *krange,
+                                      const int2 *pairs, int npairs) {
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, pairs, npairs
   if (X >= 256 && YB >= 8) {
     switch (c2r_ramp_slots(X, YB)) {
       case 1: return launch_c2r_ramp_p<STORE, 1>(A_);
@@ -1560,6 +1573,8 @@ static hipError_t launch_c2r_filter_t(const float2 *G, int X, int Y, int XB,
     }
     return hipErrorInvalidValue;
   }
+#undef A_
+#define A_ G, X, Y, XB, ndf, t0, ntilts, T, wscale, tmax, nrm_mul, slab, NYB, YB, s, tw, y0, ny, krange
   const int rb = c2r_filter_rows(X, STORE, YB);
   if (YB % rb != 0) return hipErrorInvalidValue;
   if (rb == 4) return launch_c2r_filter_it<STORE, 1>(A_);
@@ -1580,7 +1595,8 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
                                   hipStream_t s, int y0, int ny,
-                                  const int2 *krange) {
+                                  const int2 *krange, const int2 *pairs,
+                                  int npairs) {
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
@@ -1591,15 +1607,16 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
     case kF32:
       return launch_c2r_filter_t<kF32>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw, y0, ny, krange);
+                                       s, tw, y0, ny, krange, pairs, npairs);
     case kF16:
       return launch_c2r_filter_t<kF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                        wscale, tmax, nrm_mul, slab, NYB, YB,
-                                       s, tw, y0, ny, krange);
+                                       s, tw, y0, ny, krange, pairs, npairs);
     case kBF16:
       return launch_c2r_filter_t<kBF16>(G, X, Y, XB, ndf, t0, ntilts, T,
                                         wscale, tmax, nrm_mul, slab,
-                                        NYB, YB, s, tw, y0, ny, krange);
+                                        NYB, YB, s, tw, y0, ny, krange, pairs,
+                                        npairs);
   }
   return hipErrorInvalidValue;
 }

@@ -118,7 +118,9 @@ hipError_t launch_rows_c2r_filter(const float2 *G, int X, int Y, int XB,
                                   const uint32_t *tmax, float nrm_mul,
                                   int store, void *slab, int NYB, int YB,
                                   hipStream_t s, int y0 = 0, int ny = -1,
-                                  const int2 *krange = nullptr);
+                                  const int2 *krange = nullptr,
+                                  const int2 *pairs = nullptr,
+                                  int npairs = 0);
 hipError_t launch_rows_c2r(const float2 *G, int X, int Y, int XB, int ndf,
                            int ntilts, float *out, hipStream_t s);
 // tw: a twiddle table (twiddle_host_table() values) to use instead of the
@@ -147,11 +149,6 @@ size_t bp_window_entries(int W, int Z, int T);
 hipError_t launch_bp_windows(const float4 *coef, int exact, int T, int W,
                              int Z, float dscale, float doffset, int ndf,
                              int2 *win, hipStream_t s);
-// range[t] = (first, last) slab that any voxel of the W x Z grid reads at
-// tilt t (the filter passes compute only those).
-hipError_t launch_bp_slab_range(const float4 *coef, int T, int W, int Z,
-                                float dscale, float doffset, int ndf,
-                                int2 *range, hipStream_t s);
 
 // Seeded synthetic tilt series (ellipsoid phantom, optional CTF, noise).
 struct SynthArgs {
@@ -177,6 +174,8 @@ std::vector<float2> twiddle_host_table();
 
 // Per-tilt constants {c, s, offset, scale} of reconstructor.cpp:29-47.
 void tilt_geometry(size_t T, const float *angles, float centre, float *coef);
+void slab_ranges(size_t T, const float *coef, int W, int Z, float dscale,
+                 float doffset, int ndf, std::vector<int2> &out);
 // Slab selection factors of reconstructor.cpp:29-32.
 void defocus_coefficients(size_t ndf, float ps, float mn, float mx,
                           float *dscale, float *doffset);

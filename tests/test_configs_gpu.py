@@ -10,6 +10,8 @@
   volume against the FP64 chain; the filtered slabs of 8 tilts at those rows
   against the FP64 correction and filter; the backprojection of the plan's
   own decoded slabs against the oracle's.
+- 3 x 8192^2, 2 defoci, 8192 x 8 x 8192: the widest detector, four sampled
+  slices (FP32 and FP16 as above).
 - A 32-request serving mix drawn as the serving workload draws them
   (detector 512, 1024 or 2048 squared, 41-121 tilts over +-60 degrees, 1-16
   defoci, df 10000-40000 A, a seeded phantom each): every result equals its
@@ -141,6 +143,36 @@ def test_ctf_workload_sampled_slices(capi, pool, name):
     e16 = sampled.rel_l2(got["f16"], ref["f64"])
     print("FP32 vs FP32 oracle %.3g (vs FP64 %.3g), FP16 vs FP64 %.3g" % (
         e32, e32_64, e16))
+    assert e32 < 1e-5
+    assert e16 < 1e-3
+
+
+# ------------------------------------- the largest detector: 8192 x 8192 --
+
+def test_8192_detector_sampled_slices(capi, pool):
+    # The widest detector the path takes, end to end: 8192-point rows,
+    # columns and ramp (16384 points) and 32-bit spectrum offsets near their
+    # largest, on three tilts with two defoci.
+    X = Y = 8192
+    T, step, Z, ndf, ps, df = 3, 60.0, 8, 2, 1.0, 25000.0
+    print("8192^2 x %d, %d defoci" % (T, ndf), flush=True)
+    d_in, angles = series(X, Y, T, step, Z, ps, df, 200, (160.0, 1280.0),
+                          30.0, 1006)
+    d_vol = C.DeviceArray((Y, Z, X))
+    rows = numpy.linspace(0, Y - 1, 4).astype(int)
+    got = {}
+    for st in ("f32", "f16"):
+        plan, got[st] = run_rows(d_in, angles, X, Y, Z, ndf, ps, df, st,
+                                 rows, d_vol)
+        plan.destroy()
+    d_vol.free()
+    proj = d_in.to_host()
+    d_in.free()
+    ref = sampled.reference_rows(pool, proj, angles, rows, Z, ps, df, ndf,
+                                 PHI)
+    e32 = sampled.rel_l2(got["f32"], ref["f32"])
+    e16 = sampled.rel_l2(got["f16"], ref["f64"])
+    print("FP32 vs FP32 oracle %.3g, FP16 vs FP64 %.3g" % (e32, e16))
     assert e32 < 1e-5
     assert e16 < 1e-3
 
