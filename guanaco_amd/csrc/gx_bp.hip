@@ -20,7 +20,8 @@
 // kBF16 (whose error budget is set by the storage) compute p with two fused
 // multiply-adds from a table holding offset - 1/2 and the scale times the
 // normalisation the filter pass divided out, and accumulate with FP32 fused
-// multiply-adds. The per-tilt tables are read with scalar loads.
+// multiply-adds (BF16 taps rounded to FP16 first: bf16_tap). The per-tilt
+// tables are read with scalar loads.
 #include "gx_common.hpp"
 #include "gx_kernels.hpp"
 
@@ -47,6 +48,32 @@ __device__ __forceinline__ float kload(const float *p, size_t i) {
 
 template <int YB, int STORE>
 struct Taps;
+
+// BF16 taps enter the arithmetic rounded to FP16, as the staged kernel keeps
+// them in LDS (bf16_pair_to_f16): exact for magnitudes in [2^-14, 65504]
+// (eight significant bits fit FP16's eleven), and within 2^-25 below, where
+// FP16 is subnormal. The normalised slabs stay far below 65504, as they must
+// for FP16 storage. The staged kernel then interpolates with
+// v_fma_mix_f32, one instruction per tap, like FP16, where BF16 would need a
+// shift or mask before each FMA; the direct kernels round the same way so
+// that every kernel gives the same bits.
+__device__ __forceinline__ float bf16_tap(float v) {
+  return (float)(_Float16)v;
+}
+__device__ __forceinline__ uint32_t bf16_pair_to_f16(uint32_t w) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+  const f32x2 f = {__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, f16x2));
+}
+template <int STORE>
+__device__ __forceinline__ float tap_value(typename StoreT<STORE>::T v) {
+  if constexpr (STORE == kBF16) {
+    return bf16_tap(StoreT<STORE>::unpack(v));
+  } else {
+    return StoreT<STORE>::unpack(v);
+  }
+}
 
 template <int YB>
 struct Taps<YB, kF32> {
@@ -106,14 +133,14 @@ struct Taps {
     if constexpr (STORE == kF16) {
       return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu)));
     } else {
-      return __uint_as_float(w << 16);
+      return bf16_tap(__uint_as_float(w << 16));
     }
   }
   __device__ __forceinline__ static float hi(uint32_t w) {
     if constexpr (STORE == kF16) {
       return __half2float(__ushort_as_half((unsigned short)(w >> 16)));
     } else {
-      return __uint_as_float(w & 0xffff0000u);
+      return bf16_tap(__uint_as_float(w & 0xffff0000u));
     }
   }
 };
@@ -145,10 +172,14 @@ __device__ __forceinline__ int tap_index(float x, float z, float4 cf,
 // reconstructor.cpp:66-73: slab k = clamp(floor(h * dscale + doffset + 0.5)),
 // h = -x s - z c. floor((double)d + 0.5) without FP64: round half to even,
 // then move exact ties up.
+__device__ __forceinline__ float slab_coord(float x, float z, float4 cf,
+                                            float dscale, float doffset) {
+  const float h = -x * cf.y - z * cf.x;
+  return h * dscale + doffset;
+}
 __device__ __forceinline__ int slab_of(float x, float z, float4 cf,
                                        float dscale, float doffset, int ndf) {
-  const float h = -x * cf.y - z * cf.x;
-  const float d = h * dscale + doffset;
+  const float d = slab_coord(x, z, cf, dscale, doffset);
   float n = rintf(d);
   if (d - n == 0.5f) n += 1.0f;
   return min(max((int)n, 0), ndf - 1);
@@ -436,8 +467,8 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_rows(
       const float tt = p - fi;
       const int i0 = (int)fi;
       if (active && (unsigned)i0 < (unsigned)(N - 1)) {
-        const float v0 = ST::unpack(base[(size_t)i0 * YB + r]);
-        const float v1 = ST::unpack(base[(size_t)(i0 + 1) * YB + r]);
+        const float v0 = tap_value<STORE>(base[(size_t)i0 * YB + r]);
+        const float v1 = tap_value<STORE>(base[(size_t)(i0 + 1) * YB + r]);
         if constexpr (kExact) {
           acc[r] = acc[r] + cf.w * ((1.0f - tt) * v0 + tt * v1);
         } else {
@@ -468,6 +499,8 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   using S = Staging<YB, STORE, KS>;
   using ST = typename StoreT<STORE>::T;
   constexpr bool kExact = STORE == kF32;
+  // BF16 runs are staged as FP16 (bf16_tap)
+  constexpr int kLds = STORE == kBF16 ? kF16 : STORE;
   static_assert(MULTI || KS == 1, "one slab without defocus slabs");
   __shared__ uint4 lds[2 * S::kBufWords];
 
@@ -529,10 +562,14 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
 #pragma unroll
       for (int c = 0; c < S::kCPL; ++c) {
         const int ch = lane + 64 * c;
-        if (ch < S::kChunks)
+        if (ch < S::kChunks) {
+          uint4 v = pre[i][c];
+          if constexpr (STORE == kBF16)
+            v = make_uint4(bf16_pair_to_f16(v.x), bf16_pair_to_f16(v.y),
+                           bf16_pair_to_f16(v.z), bf16_pair_to_f16(v.w));
           lds[buf * S::kBufWords + q * S::kSlotWords +
-              (ch / S::kPosWords) * S::kPosStride + ch % S::kPosWords] =
-              pre[i][c];
+              (ch / S::kPosWords) * S::kPosStride + ch % S::kPosWords] = v;
+        }
       }
     }
   };
@@ -561,8 +598,18 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
           // A tile within one slab at this tilt needs no per-voxel slab
           // choice (w is the workgroup's own entry: the branch is uniform).
           if (!(w.y & kOneSlab)) {
-            ks = slab_of(x, z, cf, dscale, doffset, ndf) - (w.y & kSlabMask);
-            ks = min(max(ks, 0), KS - 1);
+            const int kf = w.y & kSlabMask;
+            if constexpr (KS == 2) {
+              // The tile's voxels take slab kf or kf + 1 (<= ndf - 1, as the
+              // tile spans two), and floor(d + 1/2) >= kf + 1 exactly when
+              // d >= kf + 1/2: one comparison of the float d with a
+              // representable bound decides as slab_of does.
+              const float d = slab_coord(x, z, cf, dscale, doffset);
+              ks = d >= (float)kf + 0.5f ? 1 : 0;
+            } else {
+              ks = slab_of(x, z, cf, dscale, doffset, ndf) - kf;
+              ks = min(max(ks, 0), KS - 1);
+            }
           }
         }
         float tt;
@@ -571,9 +618,9 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
           const int rel = min(max(i0 - w.x, 0), kWPOS - 2);
           const uint4 *q0 = &lds[boff + (j * KS + ks) * S::kSlotWords +
                                  rel * S::kPosStride];
-          Taps<YB, STORE> tp;
+          Taps<YB, kLds> tp;
           tp.load16(q0, q0 + S::kPosStride);
-          accumulate<YB, STORE>(acc, tp, cf, tt);
+          accumulate<YB, kLds>(acc, tp, cf, tt);
         }
       }
     }
