@@ -129,6 +129,24 @@ struct Taps {
       }
     }
   }
+  // The staged kernel's word-major runs: word i of position p at q[i SW]
+  // (q at position p), position p + 1 one word on.
+  template <int SW>
+  __device__ __forceinline__ void load16_strided(const uint4 *q) {
+#pragma unroll
+    for (int i = 0; i < YB / 8; ++i) {
+      const uint4 a = q[i * SW], b = q[i * SW + 1];
+      const uint32_t wa[4] = {a.x, a.y, a.z, a.w};
+      const uint32_t wb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v0[8 * i + 2 * j] = lo(wa[j]);
+        v0[8 * i + 2 * j + 1] = hi(wa[j]);
+        v1[8 * i + 2 * j] = lo(wb[j]);
+        v1[8 * i + 2 * j + 1] = hi(wb[j]);
+      }
+    }
+  }
   __device__ __forceinline__ static float lo(uint32_t w) {
     if constexpr (STORE == kF16) {
       return __half2float(__ushort_as_half((unsigned short)(w & 0xffffu)));
@@ -256,11 +274,26 @@ struct Staging {
   static constexpr int kChunks = kWPOS * kPosWords;  // words per run
   static constexpr int kCPL = (kChunks + 63) / 64;   // words per lane per run
   static constexpr int kSPW = kNSlot / (kTX * kTZ / 64);  // runs per wave
+  // LDS-direct staging (FP16): buffer_load ... lds writes lane l of a load to
+  // slot l of a dense 1 KiB block, so a run is kRunSlots 16-byte slots, slot
+  // s < kChunks holding word s / kWPOS of position s % kWPOS. In this
+  // word-major order the reads of lanes at consecutive positions are
+  // consecutive 16-byte words, which gfx950 serves without bank conflicts,
+  // as it does the padded stride of the register-staged layout.
+  static constexpr int kRunSlots = kCPL * 64;
   static_assert(kPosBytes % 16 == 0, "a position is whole 16-byte words");
   static_assert(kTB >= 1 && kBpTiltPad % kTB == 0,
                 "batches must tile the padded tilt tables");
   static_assert(kNSlot % (kTX * kTZ / 64) == 0, "runs split evenly on waves");
 };
+
+// One 16-byte raw buffer load per lane, written to LDS at dst + 16 lane
+// instead of to registers (M0 holds dst). Out-of-range offsets write zeros.
+__device__ __forceinline__ void load_to_lds16(__amdgpu_buffer_rsrc_t rs,
+                                              uint4 *dst, int off) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(
+      rs, (__attribute__((address_space(3))) void *)dst, 16, off, 0, 0, 0);
+}
 
 // Window of one tilt over the tile's active voxels [x0, x1] x [z0, z1]:
 // smallest i0 and smallest k over the four corners, with kOneSlab set in .y
@@ -501,8 +534,15 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   constexpr bool kExact = STORE == kF32;
   // BF16 runs are staged as FP16 (bf16_tap)
   constexpr int kLds = STORE == kBF16 ? kF16 : STORE;
+  // FP16 runs go straight from memory to LDS (S::kDma)
+  constexpr bool kDma = STORE == kF16;
+  constexpr int kRunWords = kDma ? S::kRunSlots : S::kSlotWords;
+  constexpr int kBufWords = S::kNSlot * kRunWords;
   static_assert(MULTI || KS == 1, "one slab without defocus slabs");
-  __shared__ uint4 lds[2 * S::kBufWords];
+  // Two buffers as two arrays: with distinct objects the compiler can tell
+  // that reads of one do not wait for the direct loads into the other.
+  __shared__ uint4 lds_a[kBufWords];
+  __shared__ uint4 lds_b[kBufWords];
 
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
   const int bz = (int)(lb % (unsigned)nbz);
@@ -525,8 +565,19 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   // from the first lane, each run's window entry is a scalar load and its
   // buffer descriptor lives in scalar registers.
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  uint4 pre[S::kSPW][S::kCPL];
-  auto load_batch = [&](int tb) {
+  uint4 pre[kDma ? 1 : S::kSPW][kDma ? 1 : S::kCPL];
+  // direct loads: each lane's byte offset within a run past its window base
+  // (-1: the slots past the run), the same for every batch
+  int dslot[kDma ? S::kCPL : 1];
+  if constexpr (kDma) {
+#pragma unroll
+    for (int c = 0; c < S::kCPL; ++c) {
+      const int s = lane + 64 * c;
+      const int wi = s / kWPOS, p = s - wi * kWPOS;
+      dslot[c] = s < S::kChunks ? (p * YB + wi * 8) * (int)sizeof(ST) : -1;
+    }
+  }
+  auto load_batch = [&](int tb, uint4 *dst) {
 #pragma unroll
     for (int i = 0; i < S::kSPW; ++i) {
       const int q = wave_u + (kTX * kTZ / 64) * i;
@@ -545,30 +596,42 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
       const auto rs = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(ok ? src : yblock), (short)0,
           ok ? (int)(row_len * sizeof(ST)) : 0, 0x00020000);
+      if constexpr (kDma) {
+        // 64 slots of the run per load, lane l to slot l of its 1 KiB
+        const int base = w.x * YB * (int)sizeof(ST);
 #pragma unroll
-      for (int c = 0; c < S::kCPL; ++c) {
-        const int ch = lane + 64 * c;
-        const int off = (w.x * YB + ch * (16 / (int)sizeof(ST))) * (int)sizeof(ST);
-        pre[i][c] = __builtin_bit_cast(
-            uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                       rs, ch < S::kChunks ? off : -1, 0, 0));
+        for (int c = 0; c < S::kCPL; ++c)
+          load_to_lds16(rs, dst + q * kRunWords + 64 * c,
+                        dslot[c] >= 0 ? base + dslot[c] : -1);
+      } else {
+#pragma unroll
+        for (int c = 0; c < S::kCPL; ++c) {
+          const int ch = lane + 64 * c;
+          const int off =
+              (w.x * YB + ch * (16 / (int)sizeof(ST))) * (int)sizeof(ST);
+          pre[i][c] = __builtin_bit_cast(
+              uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                         rs, ch < S::kChunks ? off : -1, 0, 0));
+        }
       }
     }
   };
-  auto store_batch = [&](int buf) {
+  auto store_batch = [&](uint4 *dst) {
+    if constexpr (!kDma) {
 #pragma unroll
-    for (int i = 0; i < S::kSPW; ++i) {
-      const int q = wave + (kTX * kTZ / 64) * i;
+      for (int i = 0; i < S::kSPW; ++i) {
+        const int q = wave + (kTX * kTZ / 64) * i;
 #pragma unroll
-      for (int c = 0; c < S::kCPL; ++c) {
-        const int ch = lane + 64 * c;
-        if (ch < S::kChunks) {
-          uint4 v = pre[i][c];
-          if constexpr (STORE == kBF16)
-            v = make_uint4(bf16_pair_to_f16(v.x), bf16_pair_to_f16(v.y),
-                           bf16_pair_to_f16(v.z), bf16_pair_to_f16(v.w));
-          lds[buf * S::kBufWords + q * S::kSlotWords +
-              (ch / S::kPosWords) * S::kPosStride + ch % S::kPosWords] = v;
+        for (int c = 0; c < S::kCPL; ++c) {
+          const int ch = lane + 64 * c;
+          if (ch < S::kChunks) {
+            uint4 v = pre[i][c];
+            if constexpr (STORE == kBF16)
+              v = make_uint4(bf16_pair_to_f16(v.x), bf16_pair_to_f16(v.y),
+                             bf16_pair_to_f16(v.z), bf16_pair_to_f16(v.w));
+            dst[q * kRunWords + (ch / S::kPosWords) * S::kPosStride +
+                ch % S::kPosWords] = v;
+          }
         }
       }
     }
@@ -578,15 +641,11 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
 #pragma unroll
   for (int r = 0; r < YB; ++r) acc[r] = 0.0f;
 
-  load_batch(0);
-  store_batch(0);
-  lds_barrier();
-  const int nb = (T + S::kTB - 1) / S::kTB;
-  for (int b = 0; b < nb; ++b) {
+  // One batch: prefetch the next into `nxt`, interpolate from `cur`.
+  auto batch = [&](int b, const uint4 *cur, uint4 *nxt) {
     const int tb = b * S::kTB;
-    const bool more = b + 1 < nb;
-    if (more) load_batch(tb + S::kTB);
-    const int boff = (b & 1) * S::kBufWords;
+    const bool more = tb + S::kTB < T;
+    if (more) load_batch(tb + S::kTB, nxt);
 #pragma unroll
     for (int j = 0; j < S::kTB; ++j) {
       const int t = tb + j;
@@ -616,16 +675,37 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
         const int i0 = tap_index<kExact>(x, z, cf, tt);
         if (active && (unsigned)i0 < (unsigned)(N - 1)) {
           const int rel = min(max(i0 - w.x, 0), kWPOS - 2);
-          const uint4 *q0 = &lds[boff + (j * KS + ks) * S::kSlotWords +
-                                 rel * S::kPosStride];
           Taps<YB, kLds> tp;
-          tp.load16(q0, q0 + S::kPosStride);
+          if constexpr (kDma) {
+            // word-major run: word wi of position p in slot wi kWPOS + p
+            tp.template load16_strided<kWPOS>(
+                cur + (j * KS + ks) * kRunWords + rel);
+          } else {
+            const uint4 *q0 =
+                cur + (j * KS + ks) * kRunWords + rel * S::kPosStride;
+            tp.load16(q0, q0 + S::kPosStride);
+          }
           accumulate<YB, kLds>(acc, tp, cf, tt);
         }
       }
     }
-    if (more) store_batch((b + 1) & 1);
+    if (more) {
+      store_batch(nxt);
+      // this wave's direct loads have landed; the barrier then covers the
+      // other waves' too
+      if constexpr (kDma) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    }
     lds_barrier();
+  };
+
+  load_batch(0, lds_a);
+  store_batch(lds_a);
+  if constexpr (kDma) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  lds_barrier();
+  const int nb = (T + S::kTB - 1) / S::kTB;
+  for (int b = 0; b < nb; b += 2) {
+    batch(b, lds_a, lds_b);
+    if (b + 1 < nb) batch(b + 1, lds_b, lds_a);
   }
   if (active) {
 #pragma unroll
@@ -637,7 +717,6 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
 }
 
 // ------------------------------------------------------------ launchers ---
-
 template <int YB, int STORE, bool MULTI, int KS>
 static hipError_t launch_bp_lds_k(const BpArgs &a, hipStream_t s) {
   using ST = typename StoreT<STORE>::T;
