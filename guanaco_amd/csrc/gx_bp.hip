@@ -91,6 +91,18 @@ struct Taps<YB, kF32> {
       v1[4 * i + 2] = __uint_as_float(b.z); v1[4 * i + 3] = __uint_as_float(b.w);
     }
   }
+  // word-major runs of the staged kernel: word i of position p at q[i SW]
+  template <int SW>
+  __device__ __forceinline__ void load16_strided(const uint4 *q) {
+#pragma unroll
+    for (int i = 0; i < YB / 4; ++i) {
+      const uint4 a = q[i * SW], b = q[i * SW + 1];
+      v0[4 * i] = __uint_as_float(a.x); v0[4 * i + 1] = __uint_as_float(a.y);
+      v0[4 * i + 2] = __uint_as_float(a.z); v0[4 * i + 3] = __uint_as_float(a.w);
+      v1[4 * i] = __uint_as_float(b.x); v1[4 * i + 1] = __uint_as_float(b.y);
+      v1[4 * i + 2] = __uint_as_float(b.z); v1[4 * i + 3] = __uint_as_float(b.w);
+    }
+  }
   __device__ __forceinline__ void load(const float *p) {
     if constexpr (YB % 4 == 0) {
       const uint4 *q = (const uint4 *)__builtin_assume_aligned(p, 16);
@@ -274,10 +286,10 @@ struct Staging {
   static constexpr int kChunks = kWPOS * kPosWords;  // words per run
   static constexpr int kCPL = (kChunks + 63) / 64;   // words per lane per run
   static constexpr int kSPW = kNSlot / (kTX * kTZ / 64);  // runs per wave
-  // LDS-direct staging (FP16): buffer_load ... lds writes lane l of a load to
-  // slot l of a dense 1 KiB block, so a run is kRunSlots 16-byte slots, slot
-  // s < kChunks holding word s / kWPOS of position s % kWPOS. In this
-  // word-major order the reads of lanes at consecutive positions are
+  // LDS-direct staging (FP16, FP32): buffer_load ... lds writes lane l of a
+  // load to slot l of a dense 1 KiB block, so a run is kRunSlots 16-byte
+  // slots, slot s < kChunks holding word s / kWPOS of position s % kWPOS. In
+  // this word-major order the reads of lanes at consecutive positions are
   // consecutive 16-byte words, which gfx950 serves without bank conflicts,
   // as it does the padded stride of the register-staged layout.
   static constexpr int kRunSlots = kCPL * 64;
@@ -534,8 +546,9 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   constexpr bool kExact = STORE == kF32;
   // BF16 runs are staged as FP16 (bf16_tap)
   constexpr int kLds = STORE == kBF16 ? kF16 : STORE;
-  // FP16 runs go straight from memory to LDS (S::kDma)
-  constexpr bool kDma = STORE == kF16;
+  // FP16 and FP32 runs go straight from memory to LDS (S::kRunSlots); BF16
+  // runs pass through registers to be converted
+  constexpr bool kDma = STORE != kBF16;
   constexpr int kRunWords = kDma ? S::kRunSlots : S::kSlotWords;
   constexpr int kBufWords = S::kNSlot * kRunWords;
   static_assert(MULTI || KS == 1, "one slab without defocus slabs");
@@ -574,7 +587,9 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
     for (int c = 0; c < S::kCPL; ++c) {
       const int s = lane + 64 * c;
       const int wi = s / kWPOS, p = s - wi * kWPOS;
-      dslot[c] = s < S::kChunks ? (p * YB + wi * 8) * (int)sizeof(ST) : -1;
+      dslot[c] = s < S::kChunks
+                     ? (p * YB + wi * (16 / (int)sizeof(ST))) * (int)sizeof(ST)
+                     : -1;
     }
   }
   auto load_batch = [&](int tb, uint4 *dst) {
