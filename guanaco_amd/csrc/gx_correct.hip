@@ -293,10 +293,13 @@ __device__ __forceinline__ float mask_value(M m) {
   }
 }
 
-// CW columns per workgroup (WPE waves per SIMD bound the registers). With
-// CW < CB the grid is 1D over (column part, column block, tilt), remapped so
-// that each XCD takes consecutive ones: the parts of a column block, which
-// share its 32-byte row segments, meet in one L2. The launches use CW = CB
+// CW columns per workgroup (WPE waves per SIMD bound the registers). The
+// grid is 1D over (column part, tilt, column block), remapped so that each
+// XCD takes consecutive ones: the parts of a column block, which share its
+// 32-byte row segments, meet in one L2, and so do the launch's tilts of one
+// column block, which read the same flip masks (every defocus of the block:
+// ndf Y CB bytes), so that the masks come from HBM about once per launch
+// rather than once per tilt. The launches use CW = CB
 // up to 2048 points: a wave then stores 16 rows x 4 columns, whole 128-byte
 // lines of G. With one column per workgroup at 4096 points the partial-line
 // stores doubled the bytes written to HBM (4.5 GB per tilt at 32 defoci,
@@ -324,17 +327,12 @@ __attribute__((amdgpu_waves_per_eu(WPE))) void k_cols(
   static_assert(CB % CW == 0, "whole column blocks");
   __shared__ float2 s[fft_lds_len(Y, E) * CW];
   const int cl = threadIdx.x % CW, tid = threadIdx.x / CW;
-  int cb, t, ci;
-  if constexpr (H == 1) {
-    cb = blockIdx.x;
-    t = blockIdx.y;
-    ci = cl;
-  } else {
-    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
-    ci = (int)(lb % H) * CW + cl;
-    cb = (int)((lb / H) % (unsigned)XB);
-    t = (int)(lb / (H * (unsigned)XB));
-  }
+  // (column part, tilt, column block), column part fastest
+  const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned nt = gridDim.x / (H * (unsigned)XB);  // tilts of the launch
+  const int ci = (int)(lb % H) * CW + cl;
+  const int t = (int)((lb / H) % nt);
+  const int cb = (int)(lb / (H * nt));
   const float2 *src = F + ((size_t)t * XB + cb) * Y * CB;
   typename Fft<Y, E, -1, CW>::Twiddles twy;  // shared by every transform here
   twy.load(tid, tw);
@@ -1405,7 +1403,7 @@ static hipError_t launch_cols_t(const float2 *F, const M *mask, float2 *G,
   hipError_t err;
   const float2 *tw = twiddles(&err);
   if (err != hipSuccess) return err;
-  dim3 grid(XB, ntilts);
+  const dim3 grid(XB * ntilts);  // one column part per column block
   switch (Y) {
 #define C_(n)                                                              \
   case n:                                                                  \
