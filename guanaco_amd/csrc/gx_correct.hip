@@ -298,8 +298,9 @@ __device__ __forceinline__ float mask_value(M m) {
 // XCD takes consecutive ones: the parts of a column block, which share its
 // 32-byte row segments, meet in one L2, and so do the launch's tilts of one
 // column block, which read the same flip masks (every defocus of the block:
-// ndf Y CB bytes), so that the masks come from HBM about once per launch
-// rather than once per tilt. The launches use CW = CB
+// ndf Y CB bytes), so that most of them find the masks in L2 (fetched bytes
+// per 7-tilt launch of 4096-point columns with 32 defoci: 1.86 -> 0.91 GB,
+// 3.25 -> 3.10 ms). The launches use CW = CB
 // up to 2048 points: a wave then stores 16 rows x 4 columns, whole 128-byte
 // lines of G. With one column per workgroup at 4096 points the partial-line
 // stores doubled the bytes written to HBM (4.5 GB per tilt at 32 defoci,
