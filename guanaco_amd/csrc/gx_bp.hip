@@ -265,12 +265,10 @@ hipError_t launch_bp_coefficients(const float4 *coef, const uint32_t *tmax,
 // bound (non-finite angles, which the host rejects) can only give wrong
 // values, never an access outside the buffer.
 //
-// In LDS a position's YB values are followed by 16 bytes of padding: the
-// stride is an odd number of 16-byte words, so the 16-byte reads of lanes at
-// consecutive positions fall in distinct banks.
-//
-// Two LDS buffers: the global loads of batch b + 1 are issued before batch b
-// is interpolated and written to LDS after it, one barrier per batch.
+// The runs are loaded straight into LDS (buffer_load ... lds), not through
+// registers; BF16 runs are then converted to FP16 in place by the lanes that
+// loaded them (bf16_tap). Two LDS buffers: the loads of batch b + 1 are
+// issued before batch b is interpolated, one barrier per batch.
 constexpr int kWPOS = 40;
 
 template <int YB, int STORE, int KS>
@@ -278,21 +276,18 @@ struct Staging {
   using T = typename StoreT<STORE>::T;
   static constexpr int kPosBytes = YB * (int)sizeof(T);
   static constexpr int kPosWords = kPosBytes / 16;      // 16-byte words
-  static constexpr int kPosStride = kPosWords + 1;      // in LDS, padded
   static constexpr int kTB = 256 / (kPosBytes * KS);    // tilts per batch
   static constexpr int kNSlot = kTB * KS;  // (tilt, slab) runs per batch
-  static constexpr int kSlotWords = kWPOS * kPosStride;
-  static constexpr int kBufWords = kNSlot * kSlotWords;
   static constexpr int kChunks = kWPOS * kPosWords;  // words per run
   static constexpr int kCPL = (kChunks + 63) / 64;   // words per lane per run
   static constexpr int kSPW = kNSlot / (kTX * kTZ / 64);  // runs per wave
-  // LDS-direct staging (FP16, FP32): buffer_load ... lds writes lane l of a
-  // load to slot l of a dense 1 KiB block, so a run is kRunSlots 16-byte
-  // slots, slot s < kChunks holding word s / kWPOS of position s % kWPOS. In
-  // this word-major order the reads of lanes at consecutive positions are
-  // consecutive 16-byte words, which gfx950 serves without bank conflicts,
-  // as it does the padded stride of the register-staged layout.
+  // buffer_load ... lds writes lane l of a load to slot l of a dense 1 KiB
+  // block, so a run is kRunSlots 16-byte slots, slot s < kChunks holding
+  // word s / kWPOS of position s % kWPOS. In this word-major order the reads
+  // of lanes at consecutive positions are consecutive 16-byte words, which
+  // gfx950 serves without bank conflicts (as it does a padded odd stride).
   static constexpr int kRunSlots = kCPL * 64;
+  static constexpr int kBufWords = kNSlot * kRunSlots;
   static_assert(kPosBytes % 16 == 0, "a position is whole 16-byte words");
   static_assert(kTB >= 1 && kBpTiltPad % kTB == 0,
                 "batches must tile the padded tilt tables");
@@ -546,16 +541,11 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   constexpr bool kExact = STORE == kF32;
   // BF16 runs are staged as FP16 (bf16_tap)
   constexpr int kLds = STORE == kBF16 ? kF16 : STORE;
-  // FP16 and FP32 runs go straight from memory to LDS (S::kRunSlots); BF16
-  // runs pass through registers to be converted
-  constexpr bool kDma = STORE != kBF16;
-  constexpr int kRunWords = kDma ? S::kRunSlots : S::kSlotWords;
-  constexpr int kBufWords = S::kNSlot * kRunWords;
   static_assert(MULTI || KS == 1, "one slab without defocus slabs");
   // Two buffers as two arrays: with distinct objects the compiler can tell
   // that reads of one do not wait for the direct loads into the other.
-  __shared__ uint4 lds_a[kBufWords];
-  __shared__ uint4 lds_b[kBufWords];
+  __shared__ uint4 lds_a[S::kBufWords];
+  __shared__ uint4 lds_b[S::kBufWords];
 
   const unsigned lb = xcd_remap(blockIdx.x, nblocks);
   const int bz = (int)(lb % (unsigned)nbz);
@@ -578,19 +568,16 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
   // from the first lane, each run's window entry is a scalar load and its
   // buffer descriptor lives in scalar registers.
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-  uint4 pre[kDma ? 1 : S::kSPW][kDma ? 1 : S::kCPL];
-  // direct loads: each lane's byte offset within a run past its window base
-  // (-1: the slots past the run), the same for every batch
-  int dslot[kDma ? S::kCPL : 1];
-  if constexpr (kDma) {
+  // each lane's byte offset within a run past its window base (-1: the
+  // slots past the run), the same for every batch
+  int dslot[S::kCPL];
 #pragma unroll
-    for (int c = 0; c < S::kCPL; ++c) {
-      const int s = lane + 64 * c;
-      const int wi = s / kWPOS, p = s - wi * kWPOS;
-      dslot[c] = s < S::kChunks
-                     ? (p * YB + wi * (16 / (int)sizeof(ST))) * (int)sizeof(ST)
-                     : -1;
-    }
+  for (int c = 0; c < S::kCPL; ++c) {
+    const int s = lane + 64 * c;
+    const int wi = s / kWPOS, p = s - wi * kWPOS;
+    dslot[c] = s < S::kChunks
+                   ? (p * YB + wi * (16 / (int)sizeof(ST))) * (int)sizeof(ST)
+                   : -1;
   }
   auto load_batch = [&](int tb, uint4 *dst) {
 #pragma unroll
@@ -605,48 +592,33 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
       const ST *src = yblock + (size_t)t * tstride + (size_t)k * kstride;
       // One range-checked raw buffer per run: its N positions of YB values.
       // A load at an offset outside [0, N * YB * sizeof(T)) returns zeros
-      // and touches no memory, so positions outside [0, N), the chunks past
+      // and touches no memory, so positions outside [0, N), the slots past
       // the window (offset -1) and runs past the last tilt or slab (no
       // records) all read as zero without branches.
       const auto rs = __builtin_amdgcn_make_buffer_rsrc(
           (void *)(ok ? src : yblock), (short)0,
           ok ? (int)(row_len * sizeof(ST)) : 0, 0x00020000);
-      if constexpr (kDma) {
-        // 64 slots of the run per load, lane l to slot l of its 1 KiB
-        const int base = w.x * YB * (int)sizeof(ST);
+      // 64 slots of the run per load, lane l to slot l of its 1 KiB
+      const int base = w.x * YB * (int)sizeof(ST);
 #pragma unroll
-        for (int c = 0; c < S::kCPL; ++c)
-          load_to_lds16(rs, dst + q * kRunWords + 64 * c,
-                        dslot[c] >= 0 ? base + dslot[c] : -1);
-      } else {
-#pragma unroll
-        for (int c = 0; c < S::kCPL; ++c) {
-          const int ch = lane + 64 * c;
-          const int off =
-              (w.x * YB + ch * (16 / (int)sizeof(ST))) * (int)sizeof(ST);
-          pre[i][c] = __builtin_bit_cast(
-              uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                         rs, ch < S::kChunks ? off : -1, 0, 0));
-        }
-      }
+      for (int c = 0; c < S::kCPL; ++c)
+        load_to_lds16(rs, dst + q * S::kRunSlots + 64 * c,
+                      dslot[c] >= 0 ? base + dslot[c] : -1);
     }
   };
-  auto store_batch = [&](uint4 *dst) {
-    if constexpr (!kDma) {
+  // After this wave's loads have landed: BF16 slots to FP16, each by the
+  // lane that loaded it (no other wave touches them before the barrier).
+  auto convert_batch = [&](uint4 *dst) {
+    if constexpr (STORE == kBF16) {
 #pragma unroll
       for (int i = 0; i < S::kSPW; ++i) {
         const int q = wave + (kTX * kTZ / 64) * i;
 #pragma unroll
         for (int c = 0; c < S::kCPL; ++c) {
-          const int ch = lane + 64 * c;
-          if (ch < S::kChunks) {
-            uint4 v = pre[i][c];
-            if constexpr (STORE == kBF16)
-              v = make_uint4(bf16_pair_to_f16(v.x), bf16_pair_to_f16(v.y),
-                             bf16_pair_to_f16(v.z), bf16_pair_to_f16(v.w));
-            dst[q * kRunWords + (ch / S::kPosWords) * S::kPosStride +
-                ch % S::kPosWords] = v;
-          }
+          uint4 *d = dst + q * S::kRunSlots + lane + 64 * c;
+          const uint4 v = *d;
+          *d = make_uint4(bf16_pair_to_f16(v.x), bf16_pair_to_f16(v.y),
+                          bf16_pair_to_f16(v.z), bf16_pair_to_f16(v.w));
         }
       }
     }
@@ -690,32 +662,26 @@ __global__ __launch_bounds__(kTX *kTZ) void k_bp_lds(
         const int i0 = tap_index<kExact>(x, z, cf, tt);
         if (active && (unsigned)i0 < (unsigned)(N - 1)) {
           const int rel = min(max(i0 - w.x, 0), kWPOS - 2);
+          // word-major run: word wi of position p in slot wi kWPOS + p
           Taps<YB, kLds> tp;
-          if constexpr (kDma) {
-            // word-major run: word wi of position p in slot wi kWPOS + p
-            tp.template load16_strided<kWPOS>(
-                cur + (j * KS + ks) * kRunWords + rel);
-          } else {
-            const uint4 *q0 =
-                cur + (j * KS + ks) * kRunWords + rel * S::kPosStride;
-            tp.load16(q0, q0 + S::kPosStride);
-          }
+          tp.template load16_strided<kWPOS>(cur + (j * KS + ks) * S::kRunSlots +
+                                            rel);
           accumulate<YB, kLds>(acc, tp, cf, tt);
         }
       }
     }
     if (more) {
-      store_batch(nxt);
-      // this wave's direct loads have landed; the barrier then covers the
-      // other waves' too
-      if constexpr (kDma) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      // this wave's loads have landed; the barrier then covers the other
+      // waves' too
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      convert_batch(nxt);
     }
     lds_barrier();
   };
 
   load_batch(0, lds_a);
-  store_batch(lds_a);
-  if constexpr (kDma) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  convert_batch(lds_a);
   lds_barrier();
   const int nb = (T + S::kTB - 1) / S::kTB;
   for (int b = 0; b < nb; b += 2) {
