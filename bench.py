@@ -797,8 +797,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256,
                     help="requests of the serving workload (0: skip)")
     ap.add_argument("--serve-inflight", type=int, default=4)
-    ap.add_argument("--served", type=int, default=8,
-                    help="requests of the served path of c1/c2")
+    ap.add_argument("--served", type=int, default=32,
+                    help="requests of the served path of c1/c2 (32: the "
+                         "first and last pair, which overlap nothing, weigh "
+                         "little)")
     ap.add_argument("--served-inflight", type=int, default=2)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")

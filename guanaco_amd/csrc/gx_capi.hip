@@ -1135,9 +1135,10 @@ int plan_upload(gx_volume_plan *p, hipStream_t s) {
   GX_HIP(hipMemcpyAsync(p->ramp.p, p->h_ramp.data(),
                         sizeof(float) * p->h_ramp.size(),
                         hipMemcpyHostToDevice, s));
-  // rows past Y in the last row block are read by the backprojector and
-  // discarded; keep them finite
-  GX_HIP(launch_fill_u32(p->slab.as<uint32_t>(), 0u, p->slab_bytes / 4, s));
+  // The slab buffer is not cleared: the filter writes every row below Y of
+  // every slab the volume reads, and what else the backprojector reads (the
+  // rows of the last row block past Y, the runs of slabs a tile does not use)
+  // reaches only accumulators of rows it discards or LDS it never reads.
   if (p->ctf) {
     GX_HIP(hipMemcpyAsync(p->defocus.p, p->h_defocus.data(),
                           sizeof(double) * p->ndf, hipMemcpyHostToDevice, s));
