@@ -258,7 +258,9 @@ int gx_volume_plan_get_backprojector(gx_volume_plan *plan, int *kind,
 /* The plan's filtered slabs: [k][t][y/YB][x][y%YB], element size by storage.
  * Plans with several defocus slabs compute, at each tilt t, only the slabs
  * [first[t], last[t]] that some voxel of the volume reads (no reference
- * counterpart: the reference corrects every slab); the others hold no data. */
+ * counterpart: the reference corrects every slab); the others, and the rows
+ * of the last row block past the plan's rows, hold whatever the buffer held
+ * (the buffer is not cleared). */
 int gx_volume_slabs(gx_volume_plan *plan, void **d_slab, size_t *bytes,
                     int *rows_per_block);
 /* first[t], last[t] for every tilt (host arrays of num_angles ints; 0 and 0
