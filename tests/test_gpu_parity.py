@@ -434,6 +434,9 @@ BP_KIND_CASES = [
     (64, 16, 21, 8, 64, 200, (-60.0, 60.0), (-512.0, 512.0), 1.0),
     # narrow slabs: more than four per tile, direct kernel only
     (64, 16, 21, 16, 64, 64, (-60.0, 60.0), (-32.0, 32.0), 1.0),
+    # a detector narrower than a staged run (40 positions) and the volume,
+    # a partial last row block
+    (24, 10, 29, 1, 20, 56, (-60.0, 60.0), (0.0, 0.0), 1.0),
 ]
 
 
@@ -497,7 +500,7 @@ def test_staged_and_direct_bp_bitexact_vs_oracle(capi, case):
 
 
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
-@pytest.mark.parametrize("case", BP_KIND_CASES[:4])
+@pytest.mark.parametrize("case", BP_KIND_CASES[:4] + BP_KIND_CASES[5:])
 def test_staged_bp_equals_direct_reduced_storage(capi, case, storage):
     X, Y, T, ndf, Z, W = case[:6]
     plan, sino, angles = sinogram_plan(case, storage)
