@@ -518,6 +518,36 @@ def test_staged_bp_equals_direct_reduced_storage(capi, case, storage):
     plan.destroy()
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("case", [BP_KIND_CASES[0], BP_KIND_CASES[2],
+                                  BP_KIND_CASES[5]])
+def test_unwritten_slab_memory_never_reaches_the_volume(capi, case, storage):
+    # A plan does not clear its slab buffer: the rows of the last row block
+    # past the detector's rows and the slabs no voxel reads at a tilt keep
+    # whatever the buffer held. Filled with NaN before the filter pass, they
+    # must leave every voxel, from either backprojector, bit for bit as it is
+    # when the buffer held zeros.
+    X, Y, T, ndf, Z, W = case[:6]
+    out = {}
+    for fill in (0, 0x7fc00000):
+        plan, sino, angles = sinogram_plan(case, storage)
+        p, n, _ = plan.slabs()
+        C.call("gx_fill_u32", p, fill, n // 4, None)
+        d_in = C.DeviceArray.from_host(sino)
+        slots = plan.backprojector()[1]
+        for k in (("staged", "direct") if slots else ("direct",)):
+            plan.set_backprojector(k)
+            d_vol = C.DeviceArray((Y, Z, W))
+            C.call("gx_fill_u32", d_vol.ptr, 0x7fc00000, Y * Z * W, None)
+            plan.run(d_in, d_vol)
+            C.call("gx_device_synchronize")
+            out[fill, k] = d_vol.to_host()
+        plan.destroy()
+    for (fill, k), v in out.items():
+        assert numpy.isfinite(v).all(), (fill, k)
+        numpy.testing.assert_array_equal(v, out[0, k], err_msg=k)
+
+
 def slab_rows_16(plan, case, ys, storage):
     """slab_rows for 16-bit slabs, decoded to float32."""
     X, Y, T, ndf = case[:4]
