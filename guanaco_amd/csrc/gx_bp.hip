@@ -78,30 +78,25 @@ __device__ __forceinline__ float tap_value(typename StoreT<STORE>::T v) {
 template <int YB>
 struct Taps<YB, kF32> {
   float v0[YB], v1[YB];
+  // 16-byte word i of each tap: rows 4i .. 4i + 3
+  __device__ __forceinline__ void unpack(int i, uint4 a, uint4 b) {
+    v0[4 * i] = __uint_as_float(a.x); v0[4 * i + 1] = __uint_as_float(a.y);
+    v0[4 * i + 2] = __uint_as_float(a.z); v0[4 * i + 3] = __uint_as_float(a.w);
+    v1[4 * i] = __uint_as_float(b.x); v1[4 * i + 1] = __uint_as_float(b.y);
+    v1[4 * i + 2] = __uint_as_float(b.z); v1[4 * i + 3] = __uint_as_float(b.w);
+  }
   // The YB values of each tap as YB / 4 16-byte words at q0 and at q1
-  // (global memory or LDS).
+  // (global memory).
   __device__ __forceinline__ void load16(const uint4 *q0, const uint4 *q1) {
     static_assert(YB % 4 == 0, "16-byte taps need YB % 4 == 0");
 #pragma unroll
-    for (int i = 0; i < YB / 4; ++i) {
-      const uint4 a = q0[i], b = q1[i];
-      v0[4 * i] = __uint_as_float(a.x); v0[4 * i + 1] = __uint_as_float(a.y);
-      v0[4 * i + 2] = __uint_as_float(a.z); v0[4 * i + 3] = __uint_as_float(a.w);
-      v1[4 * i] = __uint_as_float(b.x); v1[4 * i + 1] = __uint_as_float(b.y);
-      v1[4 * i + 2] = __uint_as_float(b.z); v1[4 * i + 3] = __uint_as_float(b.w);
-    }
+    for (int i = 0; i < YB / 4; ++i) unpack(i, q0[i], q1[i]);
   }
   // word-major runs of the staged kernel: word i of position p at q[i SW]
   template <int SW>
   __device__ __forceinline__ void load16_strided(const uint4 *q) {
 #pragma unroll
-    for (int i = 0; i < YB / 4; ++i) {
-      const uint4 a = q[i * SW], b = q[i * SW + 1];
-      v0[4 * i] = __uint_as_float(a.x); v0[4 * i + 1] = __uint_as_float(a.y);
-      v0[4 * i + 2] = __uint_as_float(a.z); v0[4 * i + 3] = __uint_as_float(a.w);
-      v1[4 * i] = __uint_as_float(b.x); v1[4 * i + 1] = __uint_as_float(b.y);
-      v1[4 * i + 2] = __uint_as_float(b.z); v1[4 * i + 3] = __uint_as_float(b.w);
-    }
+    for (int i = 0; i < YB / 4; ++i) unpack(i, q[i * SW], q[i * SW + 1]);
   }
   __device__ __forceinline__ void load(const float *p) {
     if constexpr (YB % 4 == 0) {
@@ -126,38 +121,28 @@ struct Taps {
     const uint4 *q = (const uint4 *)__builtin_assume_aligned(p, 16);
     load16(q, q + YB / 8);
   }
+  // 16-byte word i of each tap: rows 8i .. 8i + 7, two to a 32-bit word
+  __device__ __forceinline__ void unpack(int i, uint4 a, uint4 b) {
+    const uint32_t wa[4] = {a.x, a.y, a.z, a.w};
+    const uint32_t wb[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v0[8 * i + 2 * j] = lo(wa[j]);
+      v0[8 * i + 2 * j + 1] = hi(wa[j]);
+      v1[8 * i + 2 * j] = lo(wb[j]);
+      v1[8 * i + 2 * j + 1] = hi(wb[j]);
+    }
+  }
   __device__ __forceinline__ void load16(const uint4 *q0, const uint4 *q1) {
 #pragma unroll
-    for (int i = 0; i < YB / 8; ++i) {
-      const uint4 a = q0[i], b = q1[i];
-      const uint32_t wa[4] = {a.x, a.y, a.z, a.w};
-      const uint32_t wb[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v0[8 * i + 2 * j] = lo(wa[j]);
-        v0[8 * i + 2 * j + 1] = hi(wa[j]);
-        v1[8 * i + 2 * j] = lo(wb[j]);
-        v1[8 * i + 2 * j + 1] = hi(wb[j]);
-      }
-    }
+    for (int i = 0; i < YB / 8; ++i) unpack(i, q0[i], q1[i]);
   }
   // The staged kernel's word-major runs: word i of position p at q[i SW]
   // (q at position p), position p + 1 one word on.
   template <int SW>
   __device__ __forceinline__ void load16_strided(const uint4 *q) {
 #pragma unroll
-    for (int i = 0; i < YB / 8; ++i) {
-      const uint4 a = q[i * SW], b = q[i * SW + 1];
-      const uint32_t wa[4] = {a.x, a.y, a.z, a.w};
-      const uint32_t wb[4] = {b.x, b.y, b.z, b.w};
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v0[8 * i + 2 * j] = lo(wa[j]);
-        v0[8 * i + 2 * j + 1] = hi(wa[j]);
-        v1[8 * i + 2 * j] = lo(wb[j]);
-        v1[8 * i + 2 * j + 1] = hi(wb[j]);
-      }
-    }
+    for (int i = 0; i < YB / 8; ++i) unpack(i, q[i * SW], q[i * SW + 1]);
   }
   __device__ __forceinline__ static float lo(uint32_t w) {
     if constexpr (STORE == kF16) {
